@@ -1,0 +1,417 @@
+"""dragon_amd -- MI355X-native unsplit Godunov step of DRAGON.
+
+Python mirror of the reference's Grid / Boundary / Config.h API over the
+C ABI in ``include/dragon_gpu.h`` (``libdragon_gpu.so``, HIP kernels for
+gfx950).  There is no CPU fallback: importing the package without the built
+library, or stepping without a GPU, raises.
+
+Reference names kept (paths relative to the reference's DRAGON/ tree):
+``Grid1D/2D/3D`` (Hydro/Grid.hpp:40-144) with ``advance``,
+``advance_unsplit``, ``unsplit_step``, ``on_step_fail``,
+``initialize_B_fields`` and a ``boundary`` list; boundary elements
+``Outflow`` (+``Outflow.Gated``), ``Fixed``, ``Reflective``, ``Periodic``,
+``Ignore`` composed into a ``BoundaryList`` (Boundary/Boundary.hpp:160-229,
+BoundaryList.hpp); ``face_mask`` (Boundary/GhostFill.cpp:14-49).
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from dataclasses import dataclass, fields
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libdragon_gpu.so")
+
+# Config.h enums
+RIEMANN_EXACT, RIEMANN_HLL, RIEMANN_HLLC, RIEMANN_HLLD, RIEMANN_HLLE, RIEMANN_HLLX = 0, 1, 2, 3, 4, 5
+LIMITER_MINMOD, LIMITER_MC, LIMITER_VANLEER, LIMITER_SUPERBEE, LIMITER_VANALBADA = 0, 1, 2, 3, 4
+CT_CONSV_TOTAL_E, CT_CONSV_THERMAL, CT_CONSV_BETA_GATED = 0, 1, 2
+CFL_MAX, CFL_ADD = 0, 1
+FP64, MIXED = 0, 1
+
+# status codes (dragon_gpu.h)
+DG_OK, DG_ERR_UNPHYSICAL, DG_ERR_NAN, DG_ERR_RIEMANN_INPUT, DG_ERR_RIEMANN_SOLUTION = 0, 1, 2, 3, 4
+DG_ERR_CFL, DG_ERR_DT_UNDERFLOW, DG_ERR_ARG, DG_ERR_HIP = 5, 6, 7, 8
+
+# Face bits (Boundary.hpp:139-149)
+X_negative, X_positive, Y_negative, Y_positive, Z_negative, Z_positive = 1, 2, 4, 8, 16, 32
+X, Y, Z = 3, 12, 48
+
+
+class DragonError(RuntimeError):
+    """A failed call; ``code`` is a DG_* status, ``where`` the cell if any."""
+
+    def __init__(self, code, msg, where=None):
+        super().__init__(msg)
+        self.code = code
+        self.where = where
+
+
+# --------------------------------------------------------------- C types
+class _Desc(C.Structure):
+    _fields_ = [("dim", C.c_int), ("nx", C.c_int), ("ny", C.c_int), ("nz", C.c_int),
+                ("dx", C.c_double), ("dy", C.c_double), ("dz", C.c_double), ("ghosts", C.c_int),
+                ("mhd", C.c_int), ("gamma", C.c_double), ("solver", C.c_int), ("limiter", C.c_int),
+                ("muscl", C.c_int), ("ctu", C.c_int), ("ct_energy", C.c_int), ("ct_beta", C.c_double),
+                ("cfl_mode", C.c_int), ("cfl_coeff", C.c_double), ("verify_fallback", C.c_int),
+                ("fallback_try_hlle", C.c_int), ("fallback_param", C.c_double), ("hlld_safety", C.c_int),
+                ("timestep_tol", C.c_double), ("precision", C.c_int)]
+
+
+class _BC(C.Structure):
+    _fields_ = [("kind", C.c_int), ("faces", C.c_int), ("corners", C.c_int), ("gated", C.c_int),
+                ("state", C.c_double * 8)]
+
+
+class _Status(C.Structure):
+    _fields_ = [("code", C.c_int), ("i", C.c_int), ("j", C.c_int), ("k", C.c_int), ("msg", C.c_char * 192)]
+
+
+class _Stats(C.Structure):
+    _fields_ = [("steps", C.c_long), ("failures", C.c_long), ("t_advanced", C.c_double)]
+
+
+_dp = C.POINTER(C.c_double)
+_lib = None
+
+
+def lib():
+    """The loaded HIP library; raises if it was not built (no fallback)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise ImportError(f"dragon_amd: {LIB_PATH} is missing -- build it (python -c 'import __graft_entry__ as g; g.build()')")
+        L = C.CDLL(LIB_PATH)
+        vp = C.c_void_p
+        sig = {
+            "dg_create": (C.c_int, [C.POINTER(_Desc), C.POINTER(vp)]),
+            "dg_destroy": (C.c_int, [vp]),
+            "dg_ghosts": (C.c_int, [vp]),
+            "dg_cell_count": (C.c_long, [vp]),
+            "dg_face_count": (C.c_long, [vp]),
+            "dg_upload": (C.c_int, [vp, _dp, C.c_int, _dp]),
+            "dg_download": (C.c_int, [vp, _dp, C.c_int, _dp]),
+            "dg_upload_device": (C.c_int, [vp, vp, vp]),
+            "dg_download_device": (C.c_int, [vp, vp, vp]),
+            "dg_set_boundary": (C.c_int, [vp, C.POINTER(_BC), C.c_int]),
+            "dg_apply_boundary": (C.c_int, [vp]),
+            "dg_cfl_dt": (C.c_int, [vp, _dp, C.POINTER(_Status)]),
+            "dg_step": (C.c_int, [vp, C.c_double, C.POINTER(_Status)]),
+            "dg_advance": (C.c_int, [vp, C.c_double, C.c_int, C.c_long, C.POINTER(_Stats), C.POINTER(_Status)]),
+            "dg_init_B": (C.c_int, [vp]),
+            "dg_last_error": (C.c_char_p, [vp]),
+            "dg_stream": (vp, [vp]),
+            "dg_divb": (C.c_int, [vp, _dp]),
+        }
+        for n, (r, a) in sig.items():
+            f = getattr(L, n)
+            f.restype = r
+            f.argtypes = a
+        _lib = L
+    return _lib
+
+
+# ------------------------------------------------------------------ config
+@dataclass
+class Config:
+    """Config.h switches as run-time values (production defaults, Config.h:27-90)."""
+    mhd: int = 1
+    gamma: float = 5.0 / 3.0
+    solver: int = RIEMANN_HLLX
+    limiter: int = LIMITER_MC
+    muscl: int = 1
+    ctu: int = 1
+    ct_energy: int = CT_CONSV_BETA_GATED
+    ct_beta: float = 0.1
+    cfl_mode: int = CFL_ADD
+    cfl_coeff: float = 0.3
+    verify_fallback: int = -1
+    fallback_try_hlle: int = 0
+    fallback_param: float = 1.0
+    hlld_safety: int = 1
+    timestep_tol: float = 1e-14
+    precision: int = FP64
+
+
+# ---------------------------------------------------------------- boundary
+def face_mask(s: str) -> int:
+    """String face mask, e.g. "XY+" or "LR" (GhostFill.cpp:14-49)."""
+    mask = 0
+    s = s + " "
+    for a, b in zip(s, s[1:]):
+        u = a.upper()
+        if u in "XYZ":
+            sh = 2 * "XYZ".index(u)
+            mask |= (2 << sh) if b == "+" else (1 << sh) if b == "-" else (3 << sh)
+        elif u == "L":
+            mask |= X_negative
+        elif u == "R":
+            mask |= X_positive
+        elif u == "U":
+            mask |= Y_positive
+        elif u == "D":
+            mask |= Y_negative
+    return mask
+
+
+def _faces(f) -> int:
+    return face_mask(f) if isinstance(f, str) else int(f)
+
+
+class GhostFill:
+    kind = -1
+
+    def __init__(self, faces=X | Y | Z, corner_ghosts=True):
+        self.faces = _faces(faces)
+        self.corners = bool(corner_ghosts)
+        self.gated = False
+        self.state = np.zeros(8)
+
+    def get_faces(self):
+        return self.faces
+
+    def _c(self) -> _BC:
+        b = _BC()
+        b.kind, b.faces, b.corners, b.gated = self.kind, self.faces, int(self.corners), int(self.gated)
+        for q in range(8):
+            b.state[q] = float(self.state[q]) if q < len(self.state) else 0.0
+        return b
+
+    def __add__(self, other):
+        return BoundaryList(self) + other
+
+
+class Outflow(GhostFill):
+    kind = 0
+
+    def __init__(self, faces=X | Y | Z, corner_ghosts=True, gated=False):
+        super().__init__(faces, corner_ghosts)
+        self.gated = bool(gated)
+
+    @staticmethod
+    def Gated(faces=X | Y | Z, corner_ghosts=True):
+        return Outflow(faces, corner_ghosts, True)
+
+
+class Periodic(GhostFill):
+    kind = 1
+
+    def __init__(self, faces=X | Y | Z, corner_ghosts=True):
+        f = _faces(faces)
+        f = (X if f & X else 0) | (Y if f & Y else 0) | (Z if f & Z else 0)  # Periodic.cpp:14-15
+        super().__init__(f, corner_ghosts)
+
+
+class Reflective(GhostFill):
+    kind = 2
+
+
+class Fixed(GhostFill):
+    kind = 3
+
+    def __init__(self, state, faces=X | Y | Z, corner_ghosts=True):
+        super().__init__(faces, corner_ghosts)
+        self.state = np.zeros(8)
+        st = np.asarray(state, dtype=float)
+        self.state[: len(st)] = st
+
+
+class Ignore(GhostFill):
+    kind = 4
+
+
+class BoundaryList:
+    """Ordered boundary elements; uncovered faces get an implicit Outflow first."""
+
+    def __init__(self, *elements):
+        self.elements = []
+        for e in elements:
+            self.append(e)
+
+    def append(self, e):
+        if isinstance(e, BoundaryList):
+            self.elements.extend(e.elements)
+        else:
+            self.elements.append(e)
+        return self
+
+    def prepend(self, e):
+        if isinstance(e, BoundaryList):
+            self.elements[:0] = e.elements
+        else:
+            self.elements.insert(0, e)
+        return self
+
+    def clear(self):
+        self.elements = []
+
+    def __add__(self, other):
+        out = BoundaryList(*self.elements)
+        return out.append(other)
+
+    def __iadd__(self, other):
+        return self.append(other)
+
+
+# -------------------------------------------------------------------- grid
+class Grid:
+    """Device-resident grid with the reference's Grid API (Hydro/Grid.hpp:19-37)."""
+
+    def __init__(self, shape, d, ghosts, config: Config):
+        self.config = config
+        self.dim = len(shape)
+        self.shape = tuple(int(s) for s in shape)
+        desc = _Desc()
+        n = list(self.shape) + [1] * (3 - self.dim)
+        dd = list(d) + [1.0] * (3 - len(d))
+        desc.dim, desc.nx, desc.ny, desc.nz = self.dim, n[0], n[1], n[2]
+        desc.dx, desc.dy, desc.dz = dd
+        desc.ghosts = int(ghosts)
+        for f in fields(Config):
+            setattr(desc, f.name, getattr(config, f.name))
+        h = C.c_void_p()
+        rc = lib().dg_create(C.byref(desc), C.byref(h))
+        if rc:
+            raise DragonError(rc, f"dg_create failed ({rc})")
+        self._h = h
+        self.g = lib().dg_ghosts(h)
+        self.d = tuple(dd[: self.dim])
+        self.dx = dd[0]
+        if self.dim >= 2:
+            self.dy = dd[1]
+        if self.dim >= 3:
+            self.dz = dd[2]
+        self.has_B = bool(config.mhd) and self.dim >= 2
+        self._boundary = BoundaryList()
+        self._bc_pushed = None
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h:
+            lib().dg_destroy(h)
+            self._h = None
+
+    # -- geometry
+    def getGhosts(self):
+        return self.g
+
+    @property
+    def wshape(self):
+        return tuple(s + 2 * self.g for s in self.shape) + (8,)
+
+    @property
+    def bshape(self):
+        return tuple(s + 1 + 2 * self.g for s in self.shape) + (3,)
+
+    @property
+    def stream(self):
+        return lib().dg_stream(self._h)
+
+    # -- state
+    def get_w(self) -> np.ndarray:
+        a = np.zeros(self.wshape)
+        self._chk(lib().dg_download(self._h, a.ctypes.data_as(_dp), 8, None))
+        return a
+
+    def get_B(self) -> np.ndarray:
+        a = np.zeros(self.bshape)
+        if self.has_B:
+            w = np.zeros(self.wshape)
+            self._chk(lib().dg_download(self._h, w.ctypes.data_as(_dp), 8, a.ctypes.data_as(_dp)))
+        return a
+
+    def set_state(self, w: np.ndarray, B: np.ndarray | None = None):
+        w = np.ascontiguousarray(w, dtype=np.float64)
+        assert w.shape == self.wshape, (w.shape, self.wshape)
+        bp = None
+        if B is not None and self.has_B:
+            B = np.ascontiguousarray(B, dtype=np.float64)
+            assert B.shape == self.bshape
+            bp = B.ctypes.data_as(_dp)
+        self._chk(lib().dg_upload(self._h, w.ctypes.data_as(_dp), 8, bp))
+
+    def upload_device(self, w_ptr: int, B_ptr: int | None = None):
+        self._chk(lib().dg_upload_device(self._h, C.c_void_p(w_ptr), C.c_void_p(B_ptr) if B_ptr else None))
+
+    def download_device(self, w_ptr: int, B_ptr: int | None = None):
+        self._chk(lib().dg_download_device(self._h, C.c_void_p(w_ptr), C.c_void_p(B_ptr) if B_ptr else None))
+
+    # -- boundary (Grid::boundary)
+    @property
+    def boundary(self) -> BoundaryList:
+        return self._boundary
+
+    @boundary.setter
+    def boundary(self, b):
+        self._boundary = b if isinstance(b, BoundaryList) else BoundaryList(b)
+
+    def _push_bc(self):
+        els = [e._c() for e in self._boundary.elements]
+        key = [(b.kind, b.faces, b.corners, b.gated, tuple(b.state)) for b in els]
+        if key == self._bc_pushed:
+            return
+        arr = (_BC * max(1, len(els)))(*els)
+        self._chk(lib().dg_set_boundary(self._h, arr, len(els)))
+        self._bc_pushed = key
+
+    def apply_boundary(self):
+        self._push_bc()
+        self._chk(lib().dg_apply_boundary(self._h))
+
+    # -- stepping
+    def _chk(self, rc, st: _Status | None = None):
+        if rc:
+            msg = st.msg.decode() if st is not None and st.msg else (lib().dg_last_error(self._h) or b"").decode()
+            where = (st.i, st.j, st.k) if st is not None else None
+            raise DragonError(rc, msg, where)
+
+    def cfl_time(self) -> float:
+        """CFL::cfl_time (CFL.cpp:113-162)."""
+        out = np.zeros(1)
+        st = _Status()
+        self._chk(lib().dg_cfl_dt(self._h, out.ctypes.data_as(_dp), C.byref(st)), st)
+        return float(out[0])
+
+    def unsplit_step(self, dt: float):
+        """One unsplit step; raises DragonError and commits nothing on failure."""
+        st = _Status()
+        self._chk(lib().dg_step(self._h, float(dt), C.byref(st)), st)
+
+    def on_step_fail(self, e) -> bool:  # Unsplit.cpp:21-23 (root grid: retry)
+        return True
+
+    def advance_unsplit(self, dt: float, check_cfl=True, max_steps=-1):
+        """Grid::advance_unsplit (CFL.cpp:213-238); returns (steps, failed attempts)."""
+        self._push_bc()
+        st, S = _Status(), _Stats()
+        self._chk(lib().dg_advance(self._h, float(dt), int(check_cfl), int(max_steps), C.byref(S), C.byref(st)), st)
+        return int(S.steps), int(S.failures)
+
+    def advance(self, dt: float, check_cfl=True, max_steps=-1):
+        return self.advance_unsplit(dt, check_cfl, max_steps)
+
+    def initialize_B_fields(self):
+        self._push_bc()
+        self._chk(lib().dg_init_B(self._h))
+
+    def divb(self) -> float:
+        out = np.zeros(1)
+        self._chk(lib().dg_divb(self._h, out.ctypes.data_as(_dp)))
+        return float(out[0])
+
+
+class Grid1D(Grid):
+    def __init__(self, size, dx, ghosts=1, config: Config | None = None):
+        super().__init__((size,), (dx,), ghosts, config or Config(mhd=0))
+
+
+class Grid2D(Grid):
+    def __init__(self, nx, ny, dx, dy, ghosts=1, config: Config | None = None):
+        super().__init__((nx, ny), (dx, dy), ghosts, config or Config())
+
+
+class Grid3D(Grid):
+    def __init__(self, nx, ny, nz, dx, dy, dz, ghosts=2, config: Config | None = None):
+        super().__init__((nx, ny, nz), (dx, dy, dz), ghosts, config or Config())

@@ -1,0 +1,434 @@
+// dg_api.hip -- host runtime behind include/dragon_gpu.h.
+//
+// One handle = one device-resident grid + one HIP stream.  The state is
+// double-buffered: a step writes the complete next state (cells and faces),
+// and the commit is a pointer swap made only when the device failure word is
+// clear -- the all-or-nothing commit of the reference's checkpoint barrier
+// (Unsplit.cpp:101-110) without any copy.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/dragon_gpu.h"
+#include "dg_kernels.hpp"
+
+using namespace dg;
+
+struct dg_grid {
+    dg_desc d;
+    int g = 0, nf = 5;
+    Ext ce, fe;
+    Phys ph;
+    hipStream_t s = nullptr;
+    std::vector<dg_bc> bcs;
+    double *W = nullptr, *Wn = nullptr;
+    double *B = nullptr, *Bn = nullptr, *Bh = nullptr;
+    double *HS = nullptr, *F = nullptr, *T = nullptr;
+    double *E0 = nullptr, *Eh = nullptr;
+    unsigned long long* dev_words = nullptr;   // [0] failure key, [1] CFL max bits, [2..3] diagnostics
+    unsigned long long* host_words = nullptr;  // pinned mirror
+    std::string err;
+};
+
+static const unsigned long long NO_ERR = ~0ull;
+
+#define HIPCHK(x)                                                              \
+    do {                                                                       \
+        hipError_t e_ = (x);                                                   \
+        if (e_ != hipSuccess) {                                                \
+            if (g) g->err = std::string(#x) + ": " + hipGetErrorString(e_);   \
+            return DG_ERR_HIP;                                                 \
+        }                                                                      \
+    } while (0)
+
+static int fail(dg_grid* g, int code, const std::string& m) {
+    if (g) g->err = m;
+    return code;
+}
+static void set_status(dg_status* st, int code, int i, int j, int k, const std::string& m) {
+    if (!st) return;
+    st->code = code; st->i = i; st->j = j; st->k = k;
+    snprintf(st->msg, sizeof st->msg, "%s", m.c_str());
+}
+
+static Phys make_phys(const dg_desc& d) {
+    Phys p{};
+    const double g = d.gamma;
+    p.gamma = g; p.gm1 = g - 1.0;
+    p.k1_4pi = 0.25 / M_PI; p.k1_8pi = 0.125 / M_PI; p.sq4pi = std::sqrt(4.0 * M_PI);
+    p.Gm1_2G = (g - 1.0) / (2.0 * g); p.Gp1_2G = (g + 1.0) / (2.0 * g); p._2G_Gm1 = 2.0 * g / (g - 1.0);
+    p.Gm1_2 = (g - 1.0) / 2.0; p._2_Gm1 = 2.0 / (g - 1.0); p._2_Gp1 = 2.0 / (g + 1.0);
+    p.Gm1_Gp1 = (g - 1.0) / (g + 1.0); p.Ginv = 1.0 / g;
+    p.limiter = d.limiter; p.muscl = d.muscl; p.solver = d.solver;
+    p.verify = d.verify_fallback < 0 ? !d.mhd : d.verify_fallback != 0;
+    p.try_hlle = d.fallback_try_hlle; p.hlld_safety = d.hlld_safety; p.fallback_param = d.fallback_param;
+    p.exact_tol = 1e-12; p.exact_iters = 6; p.exact_rare_check = 1;
+    p.ct_energy = d.ct_energy; p.ct_beta = d.ct_beta;
+    p.eightpi = 8 * M_PI;
+    return p;
+}
+
+static Soa soa(double* p, long stride) { return Soa{p, stride}; }
+
+static StepCtx make_ctx(dg_grid* g, double dt) {
+    StepCtx c{};
+    c.dim = g->d.dim; c.mhd = g->d.mhd != 0; c.mixed = g->d.precision == DG_MIXED;
+    c.ctu = g->d.ctu; c.cfl_mode = g->d.cfl_mode;
+    c.ce = g->ce; c.fe = g->fe; c.ph = g->ph;
+    c.d[0] = g->d.dx; c.d[1] = g->d.dim >= 2 ? g->d.dy : 1.0; c.d[2] = g->d.dim >= 3 ? g->d.dz : 1.0;
+    c.dt = dt;
+    const long nc = g->ce.size, nfc = g->fe.size;
+    c.W = soa(g->W, nc); c.Wn = soa(g->Wn, nc);
+    for (int q = 0; q < 3; q++) {
+        c.B[q] = g->B ? g->B + q * nfc : nullptr;
+        c.Bn[q] = g->Bn ? g->Bn + q * nfc : nullptr;
+        c.Bh[q] = g->Bh ? g->Bh + q * nfc : nullptr;
+        c.E0[q] = g->E0 ? g->E0 + q * nfc : nullptr;
+        c.Eh[q] = g->Eh ? g->Eh + q * nfc : nullptr;
+    }
+    for (int q = 0; q < 2 * g->d.dim; q++) c.HS[q] = soa(g->HS + (long)q * g->nf * nc, nc);
+    for (int q = 0; q < g->d.dim; q++) c.F[q] = soa(g->F + (long)q * g->nf * nc, nc);
+    if (g->T)
+        for (int q = 0; q < 5; q++) c.T[q] = soa(g->T + (long)q * g->nf * nc, nc);
+    c.err = g->dev_words;
+    return c;
+}
+
+extern "C" {
+
+int dg_create(const dg_desc* desc, dg_grid** out) {
+    dg_grid* g = nullptr;
+    if (!desc || !out) return DG_ERR_ARG;
+    const dg_desc& d = *desc;
+    if (d.dim < 1 || d.dim > 3 || d.nx < 1 || (d.dim >= 2 && d.ny < 1) || (d.dim >= 3 && d.nz < 1)) return DG_ERR_ARG;
+    if (!(d.dx > 0) || (d.dim >= 2 && !(d.dy > 0)) || (d.dim >= 3 && !(d.dz > 0))) return DG_ERR_ARG;
+    if (d.ctu && !d.muscl) return DG_ERR_ARG;  // CTU.cpp:26-30
+    g = new dg_grid();
+    g->d = d;
+    if (g->d.timestep_tol <= 0) g->d.timestep_tol = 1e-14;
+    // validGhosts (Splitting.cpp:19-27)
+    int gh = d.ghosts;
+    if (d.mhd && d.ctu) gh = std::max(gh, 3);
+    else if (d.muscl) gh = std::max(gh, 2);
+    else gh = std::max(gh, 1);
+    g->g = gh;
+    g->nf = d.mhd ? 8 : 5;
+    g->ce = make_ext(d.dim, d.nx, d.ny, d.nz, gh);
+    g->fe = make_ext(d.dim, d.nx + 1, d.ny + 1, d.nz + 1, gh);
+    g->ph = make_phys(d);
+    const long nc = g->ce.size, nfc = g->fe.size;
+    const bool faces = d.mhd && d.dim >= 2;
+    auto alloc = [&](double** p, long n) -> bool {
+        if (n <= 0) return true;
+        return hipMalloc((void**)p, sizeof(double) * n) == hipSuccess;
+    };
+    bool ok = hipStreamCreateWithFlags(&g->s, hipStreamNonBlocking) == hipSuccess;
+    ok = ok && alloc(&g->W, 8 * nc) && alloc(&g->Wn, 8 * nc);
+    ok = ok && alloc(&g->HS, 2L * d.dim * g->nf * nc) && alloc(&g->F, (long)d.dim * g->nf * nc);
+    if (!d.mhd && d.dim == 3 && d.ctu) ok = ok && alloc(&g->T, 5L * g->nf * nc);
+    if (faces) ok = ok && alloc(&g->B, 3 * nfc) && alloc(&g->Bn, 3 * nfc) && alloc(&g->Bh, 3 * nfc) && alloc(&g->E0, 3 * nfc) &&
+                    alloc(&g->Eh, 3 * nfc);
+    ok = ok && hipMalloc((void**)&g->dev_words, 4 * sizeof(unsigned long long)) == hipSuccess;
+    ok = ok && hipHostMalloc((void**)&g->host_words, 4 * sizeof(unsigned long long), hipHostMallocDefault) == hipSuccess;
+    if (!ok) {
+        dg_destroy(g);
+        return DG_ERR_HIP;
+    }
+    ok = hipMemsetAsync(g->W, 0, sizeof(double) * 8 * nc, g->s) == hipSuccess;
+    ok = ok && hipMemsetAsync(g->Wn, 0, sizeof(double) * 8 * nc, g->s) == hipSuccess;
+    if (faces) {
+        ok = ok && hipMemsetAsync(g->B, 0, sizeof(double) * 3 * nfc, g->s) == hipSuccess;
+        ok = ok && hipMemsetAsync(g->Bn, 0, sizeof(double) * 3 * nfc, g->s) == hipSuccess;
+    }
+    ok = ok && hipStreamSynchronize(g->s) == hipSuccess;
+    if (!ok) {
+        dg_destroy(g);
+        return DG_ERR_HIP;
+    }
+    *out = g;
+    return DG_OK;
+}
+
+int dg_destroy(dg_grid* g) {
+    if (!g) return DG_OK;
+    int rc = DG_OK;
+    if (g->s && hipStreamSynchronize(g->s) != hipSuccess) rc = DG_ERR_HIP;
+    double* bufs[] = {g->W, g->Wn, g->B, g->Bn, g->Bh, g->HS, g->F, g->T, g->E0, g->Eh};
+    for (double* p : bufs)
+        if (p && hipFree(p) != hipSuccess) rc = DG_ERR_HIP;
+    if (g->dev_words && hipFree(g->dev_words) != hipSuccess) rc = DG_ERR_HIP;
+    if (g->host_words && hipHostFree(g->host_words) != hipSuccess) rc = DG_ERR_HIP;
+    if (g->s && hipStreamDestroy(g->s) != hipSuccess) rc = DG_ERR_HIP;
+    delete g;
+    return rc;
+}
+
+int dg_ghosts(const dg_grid* g) { return g ? g->g : -1; }
+long dg_cell_count(const dg_grid* g) { return g ? g->ce.size : -1; }
+long dg_face_count(const dg_grid* g) { return g && g->d.mhd && g->d.dim >= 2 ? g->fe.size : 0; }
+const char* dg_last_error(const dg_grid* g) { return g ? g->err.c_str() : "null handle"; }
+void* dg_stream(dg_grid* g) { return g ? (void*)g->s : nullptr; }
+
+int dg_upload(dg_grid* g, const double* w, int stride, const double* B) {
+    if (!g || !w || stride < g->nf) return DG_ERR_ARG;
+    const long nc = g->ce.size;
+    double* stage = nullptr;
+    HIPCHK(hipMalloc((void**)&stage, sizeof(double) * nc * stride));
+    HIPCHK(hipMemcpyAsync(stage, w, sizeof(double) * nc * stride, hipMemcpyHostToDevice, g->s));
+    launch_aos_to_soa(stage, stride, 8, soa(g->W, nc), nc, g->s);
+    if (B && g->B) {
+        const long nfc = g->fe.size;
+        double* sb = nullptr;
+        HIPCHK(hipMalloc((void**)&sb, sizeof(double) * 3 * nfc));
+        HIPCHK(hipMemcpyAsync(sb, B, sizeof(double) * 3 * nfc, hipMemcpyHostToDevice, g->s));
+        launch_aos_to_soa(sb, 3, 3, soa(g->B, nfc), nfc, g->s);
+        HIPCHK(hipStreamSynchronize(g->s));
+        HIPCHK(hipFree(sb));
+    }
+    HIPCHK(hipStreamSynchronize(g->s));
+    HIPCHK(hipFree(stage));
+    return DG_OK;
+}
+
+int dg_download(dg_grid* g, double* w, int stride, double* B) {
+    if (!g || !w || stride < g->nf) return DG_ERR_ARG;
+    const long nc = g->ce.size;
+    double* stage = nullptr;
+    HIPCHK(hipMalloc((void**)&stage, sizeof(double) * nc * stride));
+    launch_soa_to_aos(soa(g->W, nc), g->d.mhd ? 8 : 5, stage, stride, nc, g->s);
+    HIPCHK(hipMemcpyAsync(w, stage, sizeof(double) * nc * stride, hipMemcpyDeviceToHost, g->s));
+    HIPCHK(hipStreamSynchronize(g->s));
+    HIPCHK(hipFree(stage));
+    if (B && g->B) {
+        const long nfc = g->fe.size;
+        double* sb = nullptr;
+        HIPCHK(hipMalloc((void**)&sb, sizeof(double) * 3 * nfc));
+        launch_soa_to_aos(soa(g->B, nfc), 3, sb, 3, nfc, g->s);
+        HIPCHK(hipMemcpyAsync(B, sb, sizeof(double) * 3 * nfc, hipMemcpyDeviceToHost, g->s));
+        HIPCHK(hipStreamSynchronize(g->s));
+        HIPCHK(hipFree(sb));
+    }
+    return DG_OK;
+}
+
+int dg_upload_device(dg_grid* g, const double* w_soa, const double* B_soa) {
+    if (!g || !w_soa) return DG_ERR_ARG;
+    HIPCHK(hipMemcpyAsync(g->W, w_soa, sizeof(double) * 8 * g->ce.size, hipMemcpyDeviceToDevice, g->s));
+    if (B_soa && g->B) HIPCHK(hipMemcpyAsync(g->B, B_soa, sizeof(double) * 3 * g->fe.size, hipMemcpyDeviceToDevice, g->s));
+    HIPCHK(hipStreamSynchronize(g->s));
+    return DG_OK;
+}
+
+int dg_download_device(dg_grid* g, double* w_soa, double* B_soa) {
+    if (!g || !w_soa) return DG_ERR_ARG;
+    HIPCHK(hipMemcpyAsync(w_soa, g->W, sizeof(double) * 8 * g->ce.size, hipMemcpyDeviceToDevice, g->s));
+    if (B_soa && g->B) HIPCHK(hipMemcpyAsync(B_soa, g->B, sizeof(double) * 3 * g->fe.size, hipMemcpyDeviceToDevice, g->s));
+    HIPCHK(hipStreamSynchronize(g->s));
+    return DG_OK;
+}
+
+int dg_set_boundary(dg_grid* g, const dg_bc* list, int n) {
+    if (!g || n < 0 || (n > 0 && !list)) return DG_ERR_ARG;
+    g->bcs.clear();
+    for (int q = 0; q < n; q++) {
+        dg_bc b = list[q];
+        if (b.kind < 0 || b.kind > DG_BC_IGNORE) return fail(g, DG_ERR_ARG, "unknown boundary kind");
+        if (b.kind == DG_BC_PERIODIC)  // Periodic.cpp:14-15
+            b.faces = ((b.faces & 3) ? 3 : 0) | ((b.faces & 12) ? 12 : 0) | ((b.faces & 48) ? 48 : 0);
+        g->bcs.push_back(b);
+    }
+    return DG_OK;
+}
+
+// One element of the list, in the order of its reference apply(): cell passes
+// by face, then (MHD 2D/3D) face-B passes by face.
+static void apply_one(dg_grid* g, const dg_bc& b) {
+    if (b.kind == DG_BC_IGNORE) return;
+    const int D = g->d.dim, ng = g->g;
+    const bool faces = g->d.mhd && D >= 2;
+    BCArgs a{};
+    a.W = soa(g->W, g->ce.size);
+    for (int q = 0; q < 3; q++) a.B[q] = faces ? g->B + q * g->fe.size : nullptr;
+    a.ce = g->ce; a.fe = g->fe;
+    a.dim = D; a.nf = g->d.mhd ? 8 : 5; a.kind = b.kind; a.corners = b.corners; a.gated = b.gated;
+    for (int q = 0; q < 8; q++) a.st[q] = b.state[q];
+    const int n[3] = {g->ce.n[0], g->ce.n[1], g->ce.n[2]};
+    auto range = [&](int d, bool facepass, int& lo, int& hi) {
+        if (d >= D) { lo = 0; hi = 1; return; }
+        lo = b.corners ? -ng : 0;
+        hi = (b.corners ? n[d] + ng : n[d]) + (facepass ? 1 : 0);
+    };
+    for (int pass = 0; pass < (faces ? 2 : 1); pass++) {
+        const bool fp = pass == 1;
+        for (int ax = 0; ax < D; ax++) {
+            int t[2], c = 0;
+            for (int d = 0; d < 3; d++)
+                if (d != ax) t[c++] = d;
+            a.axis = ax;
+            range(t[0], fp, a.lo[0], a.hi[0]);
+            range(t[1], fp, a.lo[1], a.hi[1]);
+            const int neg = 1 << (2 * ax), pos = 2 << (2 * ax);
+            if (b.kind == DG_BC_PERIODIC) {
+                if (b.faces & (neg | pos)) { a.side = 2; launch_bc(a, fp, g->s); }
+                continue;
+            }
+            if (b.faces & neg) { a.side = 0; launch_bc(a, fp, g->s); }
+            if (b.faces & pos) { a.side = 1; launch_bc(a, fp, g->s); }
+        }
+    }
+}
+
+int dg_apply_boundary(dg_grid* g) {
+    if (!g) return DG_ERR_ARG;
+    int uncovered = 63;  // BoundaryList.hpp:104-110
+    for (auto& b : g->bcs) uncovered &= ~b.faces;
+    dg_bc imp{};
+    imp.kind = DG_BC_OUTFLOW; imp.faces = uncovered; imp.corners = 1; imp.gated = 0;
+    apply_one(g, imp);
+    for (auto& b : g->bcs) apply_one(g, b);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return fail(g, DG_ERR_HIP, hipGetErrorString(e));
+    return DG_OK;
+}
+
+int dg_cfl_dt(dg_grid* g, double* dt, dg_status* st) {
+    if (!g || !dt) return DG_ERR_ARG;
+    launch_fill_u64(g->dev_words + 1, 0ull, 1, g->s);
+    StepCtx c = make_ctx(g, 0.0);
+    launch_cfl(c, g->dev_words + 1, g->s);
+    HIPCHK(hipMemcpyAsync(g->host_words + 1, g->dev_words + 1, sizeof(unsigned long long), hipMemcpyDeviceToHost, g->s));
+    HIPCHK(hipStreamSynchronize(g->s));
+    unsigned long long bits = g->host_words[1];
+    double m;
+    memcpy(&m, &bits, sizeof m);
+    if (!(m > 0)) {
+        const char* msg = "CFL timestep failed: max signal speed is zero";
+        set_status(st, DG_ERR_CFL, 0, 0, 0, msg);
+        return fail(g, DG_ERR_CFL, msg);
+    }
+    *dt = g->d.dim == 1 ? g->d.cfl_coeff * g->d.dx / m : g->d.cfl_coeff / m;
+    set_status(st, DG_OK, 0, 0, 0, "");
+    return DG_OK;
+}
+
+static void decode(const dg_grid* g, unsigned long long key, dg_status* st, int* code, std::string* msg) {
+    const int stage = (int)(key >> 56), ec = (int)(key & 0xff);
+    const long m = (long)((key >> 8) & 0xFFFFFFFFFFFFull);
+    const Ext& e = g->ce;
+    int ijk[3] = {0, 0, 0};
+    long r = m;
+    const int D = g->d.dim;
+    const int Ny = D >= 2 ? e.n[1] + 2 * e.g : 1, Nz = D >= 3 ? e.n[2] + 2 * e.g : 1;
+    if (D == 3) { ijk[2] = (int)(r % Nz) - e.g; r /= Nz; }
+    if (D >= 2) { ijk[1] = (int)(r % Ny) - e.g; r /= Ny; }
+    ijk[0] = (int)r - e.g;
+    char b[192];
+    auto where = [&]() {
+        if (D == 1) snprintf(b, sizeof b, "(%d)", ijk[0]);
+        else if (D == 2) snprintf(b, sizeof b, "(%d,%d)", ijk[0], ijk[1]);
+        else snprintf(b, sizeof b, "(%d,%d,%d)", ijk[0], ijk[1], ijk[2]);
+        return std::string(b);
+    };
+    (void)stage;
+    switch (ec) {
+        case E_RIEMANN_INPUT: *code = DG_ERR_RIEMANN_INPUT; *msg = "Riemann Solution is unphysical because inputs were not physical"; break;
+        case E_RIEMANN_SOLUTION: *code = DG_ERR_RIEMANN_SOLUTION; *msg = "Riemann Solution to physical inputs is unphysical"; break;
+        case E_NAN: *code = DG_ERR_NAN; *msg = "\tNaN state would be produced at " + where() + "\n"; break;
+        default: *code = DG_ERR_UNPHYSICAL; *msg = "Unphysical state would be produced at " + where(); break;
+    }
+    set_status(st, *code, ijk[0], ijk[1], ijk[2], *msg);
+}
+
+int dg_step(dg_grid* g, double dt, dg_status* st) {
+    if (!g) return DG_ERR_ARG;
+    launch_fill_u64(g->dev_words, NO_ERR, 1, g->s);
+    StepCtx c = make_ctx(g, dt);
+    launch_step(c, g->s);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipMemcpyAsync(g->host_words, g->dev_words, sizeof(unsigned long long), hipMemcpyDeviceToHost, g->s));
+    HIPCHK(hipStreamSynchronize(g->s));
+    const unsigned long long key = g->host_words[0];
+    if (key != NO_ERR) {
+        int code;
+        std::string msg;
+        decode(g, key, st, &code, &msg);
+        return fail(g, code, msg);
+    }
+    std::swap(g->W, g->Wn);
+    if (g->B) std::swap(g->B, g->Bn);
+    set_status(st, DG_OK, 0, 0, 0, "");
+    return DG_OK;
+}
+
+int dg_advance(dg_grid* g, double dt, int check_cfl, long max_steps, dg_stats* stats, dg_status* st) {
+    if (!g) return DG_ERR_ARG;
+    dg_stats local{};
+    dg_stats& S = stats ? *stats : local;
+    S.steps = 0; S.failures = 0; S.t_advanced = 0;
+    const double tol = g->d.timestep_tol;
+    while (dt > tol) {
+        if (max_steps >= 0 && S.steps >= max_steps) break;
+        int rc = dg_apply_boundary(g);
+        if (rc) return rc;
+        double t1 = dt;
+        if (check_cfl) {
+            double c;
+            rc = dg_cfl_dt(g, &c, st);
+            if (rc) return rc;
+            t1 = (c < dt) ? c : dt;  // std::min(dt, cfl)
+        }
+        while (true) {
+            rc = dg_step(g, t1, st);
+            if (rc == DG_OK) break;
+            if (rc == DG_ERR_HIP || rc == DG_ERR_ARG) return rc;
+            S.failures++;
+            t1 *= 0.5;
+            if (t1 < tol) {
+                const char* msg = "Timestep has fallen below minimum. Exiting";
+                set_status(st, DG_ERR_DT_UNDERFLOW, 0, 0, 0, msg);
+                return fail(g, DG_ERR_DT_UNDERFLOW, msg);
+            }
+        }
+        dt -= t1;
+        S.t_advanced += t1;
+        S.steps++;
+    }
+    set_status(st, DG_OK, 0, 0, 0, "");
+    return DG_OK;
+}
+
+int dg_init_B(dg_grid* g) {
+    if (!g) return DG_ERR_ARG;
+    int rc = dg_apply_boundary(g);
+    if (rc) return rc;
+    if (!(g->d.mhd && g->d.dim >= 2)) return DG_OK;
+    launch_init_B(g->d.dim, soa(g->W, g->ce.size), g->B, g->ce, g->fe, g->s);
+    HIPCHK(hipStreamSynchronize(g->s));
+    return DG_OK;
+}
+
+int dg_divb(dg_grid* g, double* out) {
+    if (!g || !out) return DG_ERR_ARG;
+    *out = 0;
+    if (!(g->d.mhd && g->d.dim >= 2)) return DG_OK;
+    launch_fill_u64(g->dev_words + 2, 0ull, 2, g->s);
+    const double d[3] = {g->d.dx, g->d.dy, g->d.dz};
+    launch_divb(g->d.dim, g->B, g->ce, g->fe, d, g->dev_words + 2, g->s);
+    HIPCHK(hipMemcpyAsync(g->host_words + 2, g->dev_words + 2, 2 * sizeof(unsigned long long), hipMemcpyDeviceToHost, g->s));
+    HIPCHK(hipStreamSynchronize(g->s));
+    double dv, bm;
+    memcpy(&dv, &g->host_words[2], 8);
+    memcpy(&bm, &g->host_words[3], 8);
+    double dmin = g->d.dx;
+    if (g->d.dim >= 2) dmin = std::min(dmin, g->d.dy);
+    if (g->d.dim >= 3) dmin = std::min(dmin, g->d.dz);
+    *out = bm > 0 ? dv * dmin / bm : 0.0;
+    return DG_OK;
+}
+
+}  // extern "C"

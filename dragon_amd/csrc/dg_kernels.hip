@@ -1,0 +1,892 @@
+// dg_kernels.hip -- CDNA4 kernels for DRAGON's unsplit Godunov step.
+//
+// One kernel per loop nest of the reference step, each over exactly the index
+// range the reference uses, reading and writing SoA FP64 fields in HBM.  The
+// ranges are what make the FP64 form bit-compatible with the CPU path; the
+// stage order (enforced by stream order) is the reference's.
+//
+// Kernels are templated on the dimension D, on MHD, and on the arithmetic type
+// R of reconstruction and Riemann solves (double: drop-in; float: the mixed
+// form, whose state, accumulation, pressure recovery and CT stay double).
+#include "dg_kernels.hpp"
+
+namespace dg {
+
+#define KB __launch_bounds__(256)
+
+__device__ __forceinline__ long tid() { return (long)blockIdx.x * blockDim.x + threadIdx.x; }
+
+template <class R> __device__ __forceinline__ P<R> load_p(const Soa& a, long m, bool mhd) {
+    P<R> w;
+    w.rho = R(a(F_RHO, m)); w.p = R(a(F_P, m));
+    w.vx = R(a(F_VX, m)); w.vy = R(a(F_VY, m)); w.vz = R(a(F_VZ, m));
+    if (mhd) { w.Bx = R(a(F_BX, m)); w.By = R(a(F_BY, m)); w.Bz = R(a(F_BZ, m)); }
+    else { w.Bx = w.By = w.Bz = R(0); }
+    return w;
+}
+template <class R> __device__ __forceinline__ void store_p(const Soa& a, long m, const P<R>& w, bool mhd) {
+    a(F_RHO, m) = double(w.rho); a(F_P, m) = double(w.p);
+    a(F_VX, m) = double(w.vx); a(F_VY, m) = double(w.vy); a(F_VZ, m) = double(w.vz);
+    if (mhd) { a(F_BX, m) = double(w.Bx); a(F_BY, m) = double(w.By); a(F_BZ, m) = double(w.Bz); }
+}
+// Flux records use the U field order: rho, E, mx, my, mz, Bx, By, Bz.
+__device__ __forceinline__ U<double> load_u(const Soa& a, long m, bool mhd) {
+    U<double> u;
+    u.rho = a(0, m); u.E = a(1, m); u.mx = a(2, m); u.my = a(3, m); u.mz = a(4, m);
+    if (mhd) { u.Bx = a(5, m); u.By = a(6, m); u.Bz = a(7, m); } else { u.Bx = u.By = u.Bz = 0; }
+    return u;
+}
+template <class R> __device__ __forceinline__ void store_u(const Soa& a, long m, const U<R>& u, bool mhd) {
+    a(0, m) = double(u.rho); a(1, m) = double(u.E); a(2, m) = double(u.mx); a(3, m) = double(u.my); a(4, m) = double(u.mz);
+    if (mhd) { a(5, m) = double(u.Bx); a(6, m) = double(u.By); a(7, m) = double(u.Bz); }
+}
+__device__ __forceinline__ void cross(double ux, double uy, double uz, double vx, double vy, double vz, double& wx, double& wy, double& wz) {
+    wx = uy * vz - uz * vy;  // Arithmetic.cpp:359-365
+    wy = uz * vx - ux * vz;
+    wz = ux * vy - uy * vx;
+}
+
+// ===================================================================== BC
+// One thread per transverse index of a face; it runs the reference's ghost
+// loop (g = 1..ng) in order, since face-B extrapolation reads the ghost it
+// wrote one iteration earlier.  Cell passes and face passes are separate
+// launches, issued in the reference's order.
+
+
+__device__ __forceinline__ long cidx(const Ext& e, int axis, int a, int t0, int t1) {
+    // a along `axis`, (t0, t1) the other two dims in increasing order
+    int i, j, k;
+    if (axis == 0) { i = a; j = t0; k = t1; }
+    else if (axis == 1) { i = t0; j = a; k = t1; }
+    else { i = t0; j = t1; k = a; }
+    return e.at(i, j, k);
+}
+
+__device__ void copy_cell(const Soa& W, long dst, long src, int nf) {
+    for (int f = 0; f < nf; f++) W(f, dst) = W(f, src);
+}
+
+KB __global__ void k_bc_cells(BCArgs a) {
+    const long t = tid();
+    const int n1 = a.hi[0] - a.lo[0], n2 = a.hi[1] - a.lo[1];
+    if (t >= (long)n1 * n2) return;
+    const int t0 = a.lo[0] + (int)(t / n2), t1 = a.lo[1] + (int)(t % n2);
+    const Ext& e = a.ce;
+    const int n = e.n[a.axis], ng = e.g;
+    const int nf = a.nf;
+    const int vf = F_VX + a.axis, bfn = F_BX + a.axis;
+    for (int g = 1; g <= ng; g++) {
+        if (a.kind == 1) {  // Periodic (Periodic.cpp): both sides per g
+            copy_cell(a.W, cidx(e, a.axis, -g, t0, t1), cidx(e, a.axis, n - g, t0, t1), nf);
+            copy_cell(a.W, cidx(e, a.axis, n - 1 + g, t0, t1), cidx(e, a.axis, g - 1, t0, t1), nf);
+            continue;
+        }
+        const long dst = cidx(e, a.axis, a.side == 0 ? -g : n - 1 + g, t0, t1);
+        if (a.kind == 0) {  // Outflow (+gated) (Outflow.cpp)
+            copy_cell(a.W, dst, cidx(e, a.axis, a.side == 0 ? 0 : n - 1, t0, t1), nf);
+            if (a.gated) {
+                double v = a.W(vf, dst);
+                if (a.side == 0 ? v > 0 : v < 0) a.W(vf, dst) = 0;
+            }
+        } else if (a.kind == 2) {  // Reflective (Reflective.cpp)
+            copy_cell(a.W, dst, cidx(e, a.axis, a.side == 0 ? g - 1 : n - g, t0, t1), nf);
+            a.W(vf, dst) *= -1;
+            if (nf == 8) {
+                for (int f = F_BX; f <= F_BZ; f++) a.W(f, dst) *= -1;
+                a.W(bfn, dst) *= -1;
+            }
+        } else if (a.kind == 3) {  // Fixed (Fixed.cpp)
+            for (int f = 0; f < nf; f++) a.W(f, dst) = a.st[f];
+        }
+    }
+}
+
+__device__ __forceinline__ void fget(double* const B[3], long m, double v[3]) { v[0] = B[0][m]; v[1] = B[1][m]; v[2] = B[2][m]; }
+__device__ __forceinline__ void fset(double* const B[3], long m, const double v[3]) { B[0][m] = v[0]; B[1][m] = v[1]; B[2][m] = v[2]; }
+
+KB __global__ void k_bc_faces(BCArgs a) {
+    const long t = tid();
+    const int n1 = a.hi[0] - a.lo[0], n2 = a.hi[1] - a.lo[1];
+    if (t >= (long)n1 * n2) return;
+    const int t0 = a.lo[0] + (int)(t / n2), t1 = a.lo[1] + (int)(t % n2);
+    const Ext& e = a.fe;
+    const int n = a.ce.n[a.axis], ng = e.g, ax = a.axis;
+    auto at = [&](int q) { return cidx(e, ax, q, t0, t1); };
+    double v[3], u[3];
+    if (a.kind == 1) {  // Periodic
+        for (int g = 1; g <= ng; g++) {
+            fget(a.B, at(n - g), v); fset(a.B, at(-g), v);
+            fget(a.B, at(g - 1), v); fset(a.B, at(n - 1 + g), v);
+        }
+        return;
+    }
+    if (a.kind == 0) {  // Outflow: copy transverse, extrapolate normal
+        if (a.side == 0) {
+            for (int g = 1; g <= ng; g++) {
+                fget(a.B, at(0), v);
+                v[ax] = 2 * a.B[ax][at(-g + 1)] - a.B[ax][at(-g + 2)];
+                fset(a.B, at(-g), v);
+            }
+        } else {
+            for (int c = 0; c < 3; c++) if (c != ax) a.B[c][at(n)] = a.B[c][at(n - 1)];
+            for (int g = 1; g <= ng; g++) {
+                fget(a.B, at(n), v);
+                v[ax] = 2 * a.B[ax][at(n + g - 1)] - a.B[ax][at(n + g - 2)];
+                fset(a.B, at(n + g), v);
+            }
+        }
+    } else if (a.kind == 2) {  // Reflective: invert, copy normal from the mirror face
+        if (a.side == 0) {
+            for (int g = 1; g <= ng; g++) {
+                fget(a.B, at(g - 1), u);
+                for (int c = 0; c < 3; c++) v[c] = u[c] * -1;
+                v[ax] = a.B[ax][at(g)];
+                fset(a.B, at(-g), v);
+            }
+        } else {
+            fget(a.B, at(n - 1), u);
+            for (int c = 0; c < 3; c++) v[c] = u[c] * -1;
+            fset(a.B, at(n), v);
+            for (int g = 1; g <= ng; g++) {
+                fget(a.B, at(n - g - 1), u);
+                for (int c = 0; c < 3; c++) v[c] = u[c] * -1;
+                v[ax] = a.B[ax][at(n - g)];
+                fset(a.B, at(n + g), v);
+            }
+        }
+    } else if (a.kind == 3) {  // Fixed
+        const double s[3] = {a.st[F_BX], a.st[F_BY], a.st[F_BZ]};
+        if (a.side == 0) {
+            a.B[ax][at(0)] = s[ax];
+            for (int g = 1; g <= ng; g++) fset(a.B, at(-g), s);
+        } else {
+            for (int g = 0; g < ng; g++) fset(a.B, at(n + g), s);
+        }
+    }
+}
+
+// ==================================================================== CFL
+// Max signal speed over cells [-1, n+1)^D (CFL.cpp:113-162); wave64
+// shuffles then one atomicMax per block on the (non-negative) FP64 bits.
+template <int D, bool MHD>
+KB __global__ void k_cfl(Soa W, Ext ce, Box box, Phys ph, int mode, double dx, double dy, double dz, unsigned long long* out) {
+    double m = 0;
+    int i, j, k;
+    if (box.point(tid(), i, j, k)) {
+        P<double> w = load_p<double>(W, ce.at(i, j, k), MHD);
+        double a = MHD ? c_fast_max(w, ph) : cs(w, ph);
+        double s = 0;
+        if (D == 1) {
+            s = dabs(w.vx) + a;
+        } else if (mode == 0) {  // CFL_MAX
+            if (dx > 1e-14) s = (dabs(w.vx) + a) / dx;
+            if (dy > 1e-14) s = dmax(s, (dabs(w.vy) + a) / dy);
+            if (D == 3 && dz > 1e-14) s = dmax(s, (dabs(w.vz) + a) / dz);
+        } else if (mode == 1) {  // CFL_ADD
+            if (dx > 1e-14) s = (dabs(w.vx) + a) / dx;
+            if (dy > 1e-14) s += (dabs(w.vy) + a) / dy;
+            if (D == 3 && dz > 1e-14) s += (dabs(w.vz) + a) / dz;
+        } else {  // power mean
+            const double p = mode;
+            if (dx > 1e-14) s = pow((dabs(w.vx) + a) / dx, p);
+            if (dy > 1e-14) s += pow((dabs(w.vy) + a) / dy, p);
+            if (D == 3 && dz > 1e-14) s += pow((dabs(w.vz) + a) / dz, p);
+            s = pow(s, 1.0 / p);
+        }
+        m = dmax(m, s);  // NaN never wins, as with std::max
+    }
+    for (int off = 32; off > 0; off >>= 1) m = dmax(m, __shfl_xor(m, off));
+    __shared__ double part[4];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    if (lane == 0) part[wave] = m;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double b = part[0];
+        for (int q = 1; q < (int)(blockDim.x >> 6); q++) b = dmax(b, part[q]);
+        if (b > 0) atomicMax(out, (unsigned long long)__double_as_longlong(b));
+    }
+}
+
+// ============================================================ half states
+// computeHalfStates_X/Y/Z + copyFaceFields (Unsplit.cpp:292-436,
+// B_fields.cpp:166-216): every cell, every axis.  Cells at the outer ghost
+// layer along an axis get the first-order copy.
+template <int D, bool MHD, class R>
+KB __global__ void k_half(Soa W, double* Bx, double* By, double* Bz, Ext ce, Ext fe, Soa HS0, Soa HS1, Soa HS2, Soa HS3,
+                          Soa HS4, Soa HS5, double dt_dx, double dt_dy, double dt_dz, Phys ph) {
+    Box box = make_box(D, -ce.g, ce.n[0] + ce.g, -ce.g, ce.n[1] + ce.g, -ce.g, ce.n[2] + ce.g);
+    int i, j, k;
+    if (!box.point(tid(), i, j, k)) return;
+    const long m = ce.at(i, j, k);
+    const int g = ce.g;
+    double bx = 0, by = 0, bz = 0;
+    if (MHD && D >= 2) {
+        bx = (Bx[fe.at(i + 1, j, k)] - Bx[fe.at(i, j, k)]) * dt_dx;
+        by = (By[fe.at(i, j + 1, k)] - By[fe.at(i, j, k)]) * dt_dy;
+        if (D == 3) bz = (Bz[fe.at(i, j, k + 1)] - Bz[fe.at(i, j, k)]) * dt_dz;
+    }
+    const int idx[3] = {i, j, k};
+    const P<R> wc = load_p<R>(W, m, MHD);
+#pragma unroll
+    for (int ax = 0; ax < D; ax++) {
+        P<R> L, Rr;
+        const bool inner = idx[ax] >= -g + 1 && idx[ax] < ce.n[ax] + g - 1;
+        if (inner) {
+            const long off = ce.s[ax];
+            P<R> wl = load_p<R>(W, m - off, MHD), wr = load_p<R>(W, m + off, MHD);
+            if (ax == 0) {
+                muscl<MHD, R>(wl, L, wc, Rr, wr, R(dt_dx), R(bx), R(by), R(bz), ph);
+            } else if (ax == 1) {
+                P<R> l, r;
+                muscl<MHD, R>(swapXY(wl), l, swapXY(wc), r, swapXY(wr), R(dt_dy), R(by), R(bx), R(bz), ph);
+                L = swapXY(l); Rr = swapXY(r);
+            } else {
+                P<R> l, r;
+                muscl<MHD, R>(swapXZ(wl), l, swapXZ(wc), r, swapXZ(wr), R(dt_dz), R(bz), R(by), R(bx), ph);
+                L = swapXZ(l); Rr = swapXZ(r);
+            }
+        } else {
+            L = wc; Rr = wc;
+        }
+        if (MHD && D >= 2) {  // normal B of the half states := face B
+            if (ax == 0) { L.Bx = R(Bx[fe.at(i, j, k)]); Rr.Bx = R(Bx[fe.at(i + 1, j, k)]); }
+            else if (ax == 1) { L.By = R(By[fe.at(i, j, k)]); Rr.By = R(By[fe.at(i, j + 1, k)]); }
+            else { L.Bz = R(Bz[fe.at(i, j, k)]); Rr.Bz = R(Bz[fe.at(i, j, k + 1)]); }
+        }
+        // the FP32 form stores half states in FP64 arrays; exact for floats
+        if (ax == 0) { store_p<R>(HS0, m, L, MHD); store_p<R>(HS1, m, Rr, MHD); }
+        else if (ax == 1) { store_p<R>(HS2, m, L, MHD); store_p<R>(HS3, m, Rr, MHD); }
+        else { store_p<R>(HS4, m, L, MHD); store_p<R>(HS5, m, Rr, MHD); }
+    }
+}
+
+// ================================================================= fluxes
+// computeFlux_X/Y/Z (Unsplit.cpp:210-254): F[c] = Riemann(R[c - e], L[c]).
+template <int D, bool MHD, class R, int AX>
+KB __global__ void k_flux(Soa HL, Soa HR, Soa F, Ext ce, Box box, double dt_d, Phys ph, unsigned long long* err, int stage) {
+    int i, j, k;
+    if (!box.point(tid(), i, j, k)) return;
+    const long m = ce.at(i, j, k);
+    P<R> l = load_p<R>(HR, m - ce.s[AX], MHD), r = load_p<R>(HL, m, MHD);
+    if (AX == 1) { l = swapXY(l); r = swapXY(r); }
+    if (AX == 2) { l = swapXZ(l); r = swapXZ(r); }
+    int fail = RF_OK;
+    U<R> f = riemann_x<MHD, R>(l, r, R(dt_d), ph, &fail);
+    if (fail) report(err, stage, (unsigned long long)m, fail == RF_BAD_INPUT ? E_RIEMANN_INPUT : E_RIEMANN_SOLUTION);
+    if (AX == 1) f = swapXY(f);
+    if (AX == 2) f = swapXZ(f);
+    store_u<R>(F, m, f, MHD);
+}
+
+// ===================================================== CTU hydro (2D/3D)
+// correctState (CTU.cpp:264-301): states along SAX corrected by the flux
+// difference of FAX.
+template <int D, class R>
+KB __global__ void k_correct(Soa HL, Soa HR, Soa F, Ext ce, Box box, int fax, double dt_dL, Phys ph) {
+    int i, j, k;
+    if (!box.point(tid(), i, j, k)) return;
+    const long m = ce.at(i, j, k);
+    U<double> t = umul<false>(usub<false>(load_u(F, m + ce.s[fax], false), load_u(F, m, false)), dt_dL);
+    P<double> l = load_p<double>(HL, m, false), r = load_p<double>(HR, m, false);
+    store_p<double>(HL, m, prim<false>(usub<false>(cons<false>(l, ph), t), ph), false);
+    store_p<double>(HR, m, prim<false>(usub<false>(cons<false>(r, ph), t), ph), false);
+}
+
+// computeCTUFlux_X/Y/Z (CTU.cpp:306-370): a flux between states corrected by
+// the transverse flux T (3D hydro, 12-solve CTU).
+template <class R, int AX>
+KB __global__ void k_ctu_flux(Soa HL, Soa HR, Soa T, Soa F, Ext ce, Box box, int tax, double dt_d, double dt_t, Phys ph,
+                              unsigned long long* err, int stage) {
+    int i, j, k;
+    if (!box.point(tid(), i, j, k)) return;
+    const long m = ce.at(i, j, k), e = ce.s[AX], te = ce.s[tax];
+    auto corr = [&](const Soa& S, long q) {
+        U<double> t = umul<false>(usub<false>(load_u(T, q + te, false), load_u(T, q, false)), dt_t);
+        return prim<false>(usub<false>(cons<false>(load_p<double>(S, q, false), ph), t), ph);
+    };
+    P<double> l = corr(HR, m - e), r = corr(HL, m);
+    P<R> lr = {R(l.rho), R(l.p), R(l.vx), R(l.vy), R(l.vz), 0, 0, 0};
+    P<R> rr = {R(r.rho), R(r.p), R(r.vx), R(r.vy), R(r.vz), 0, 0, 0};
+    if (AX == 1) { lr = swapXY(lr); rr = swapXY(rr); }
+    if (AX == 2) { lr = swapXZ(lr); rr = swapXZ(rr); }
+    int fail = RF_OK;
+    U<R> f = riemann_x<false, R>(lr, rr, R(dt_d), ph, &fail);
+    if (fail) {
+        // the reference walks these loops with the flux axis innermost
+        const int ia[3] = {i + ce.g, j + ce.g, k + ce.g};
+        const int o0 = AX == 0 ? 1 : 0, o1 = AX == 2 ? 1 : 2;
+        unsigned long long seq = ((unsigned long long)ia[o0] * 65536ull + (unsigned long long)ia[o1]) * 65536ull + ia[AX];
+        report(err, stage, seq, fail == RF_BAD_INPUT ? E_RIEMANN_INPUT : E_RIEMANN_SOLUTION);
+    }
+    if (AX == 1) f = swapXY(f);
+    if (AX == 2) f = swapXZ(f);
+    store_u<R>(F, m, f, false);
+}
+
+// ================================================================ MHD CT
+__device__ __forceinline__ double upwind_corr(double m, double fm, double fp, double rm, double rp) {  // Electric.cpp:96-101
+    if (m > 1e-18) return 0.25 * (fm - rm);
+    if (m < -1e-18) return 0.25 * (fp - rp);
+    return 0.125 * (fm + fp - rm - rp);
+}
+// Flux SoA field indices (U order)
+enum { U_RHO = 0, U_BX = 5, U_BY = 6, U_BZ = 7 };
+
+// Reference EMF of a cell: cross(w.B, w.v) (Electric.cpp:108-125); `Ws`
+// selects whether the cell state comes from W (predictor) or from the
+// half-step Eref arrays (corrector).
+struct ERef {
+    const double *x, *y, *z;
+};
+
+// 2D corner EMF (Electric.cpp:24-35, 52-71) at edge (i,j).  With `fromW`,
+// Eref is computed from the cell states (predictor); otherwise it is read.
+template <bool FROM_W>
+KB __global__ void k_emf2(Soa FX, Soa FY, Soa W, ERef ref, Ext ce, Ext fe, Box box, int upwind, double* Ex, double* Ey,
+                          double* Ez) {
+    int i, j, k;
+    if (!box.point(tid(), i, j, k)) return;
+    auto f = [&](const Soa& F, int fld, int a, int b) { return F(fld, ce.at(a, b, 0)); };
+    double ex = -f(FY, U_BZ, i, j);
+    double ey = f(FX, U_BZ, i, j);
+    double ez = (f(FY, U_BX, i - 1, j) - f(FX, U_BY, i, j - 1) + f(FY, U_BX, i, j) - f(FX, U_BY, i, j)) * 0.25;
+    if (upwind) {
+        double rx[2][2], ry[2][2], rz[2][2];  // [di][dj] for cells (i-1+di, j-1+dj)
+        for (int di = 0; di < 2; di++)
+            for (int dj = 0; dj < 2; dj++) {
+                if (FROM_W) {
+                    const long c = ce.at(i - 1 + di, j - 1 + dj, 0);
+                    cross(W(F_BX, c), W(F_BY, c), W(F_BZ, c), W(F_VX, c), W(F_VY, c), W(F_VZ, c), rx[di][dj], ry[di][dj], rz[di][dj]);
+                } else {
+                    const long c = fe.at(i - 1 + di, j - 1 + dj, 0);
+                    rx[di][dj] = ref.x[c]; ry[di][dj] = ref.y[c]; rz[di][dj] = ref.z[c];
+                }
+            }
+        ez += upwind_corr(f(FX, U_RHO, i, j), f(FY, U_BX, i - 1, j), f(FY, U_BX, i, j), rz[0][1], rz[1][1]);
+        ez += upwind_corr(f(FX, U_RHO, i, j - 1), f(FY, U_BX, i - 1, j), f(FY, U_BX, i, j), rz[0][0], rz[1][0]);
+        ez += upwind_corr(f(FY, U_RHO, i, j), -f(FX, U_BY, i, j - 1), -f(FX, U_BY, i, j), rz[1][0], rz[1][1]);
+        ez += upwind_corr(f(FY, U_RHO, i - 1, j), -f(FX, U_BY, i, j - 1), -f(FX, U_BY, i, j), rz[0][0], rz[0][1]);
+        ex += 0.25 * (-2 * f(FY, U_BZ, i, j) - rx[1][1] - rx[1][0]);
+        ey += 0.25 * (2 * f(FX, U_BZ, i, j) - ry[1][1] - ry[0][1]);
+    }
+    const long e = fe.at(i, j, 0);
+    Ex[e] = ex; Ey[e] = ey; Ez[e] = ez;
+}
+
+// 3D corner EMF (Electric.cpp:38-47, 74-100).
+template <bool FROM_W>
+KB __global__ void k_emf3(Soa FX, Soa FY, Soa FZ, Soa W, ERef ref, Ext ce, Ext fe, Box box, int upwind, double* Ex,
+                          double* Ey, double* Ez) {
+    int i, j, k;
+    if (!box.point(tid(), i, j, k)) return;
+    auto f = [&](const Soa& F, int fld, int a, int b, int c) { return F(fld, ce.at(a, b, c)); };
+    double ex = (f(FZ, U_BY, i, j - 1, k) - f(FY, U_BZ, i, j, k - 1) + f(FZ, U_BY, i, j, k) - f(FY, U_BZ, i, j, k)) * 0.25;
+    double ey = (f(FX, U_BZ, i, j, k - 1) - f(FZ, U_BX, i - 1, j, k) + f(FX, U_BZ, i, j, k) - f(FZ, U_BX, i, j, k)) * 0.25;
+    double ez = (f(FY, U_BX, i - 1, j, k) - f(FX, U_BY, i, j - 1, k) + f(FY, U_BX, i, j, k) - f(FX, U_BY, i, j, k)) * 0.25;
+    if (upwind) {
+        auto R = [&](int comp, int a, int b, int c) -> double {
+            if (FROM_W) {
+                const long q = ce.at(a, b, c);
+                double x, y, z;
+                cross(W(F_BX, q), W(F_BY, q), W(F_BZ, q), W(F_VX, q), W(F_VY, q), W(F_VZ, q), x, y, z);
+                return comp == 0 ? x : comp == 1 ? y : z;
+            }
+            const long q = fe.at(a, b, c);
+            return comp == 0 ? ref.x[q] : comp == 1 ? ref.y[q] : ref.z[q];
+        };
+        ey += upwind_corr(f(FX, U_RHO, i, j, k), -f(FZ, U_BX, i - 1, j, k), -f(FZ, U_BX, i, j, k), R(1, i - 1, j, k), R(1, i, j, k));
+        ey += upwind_corr(f(FX, U_RHO, i, j, k - 1), -f(FZ, U_BX, i - 1, j, k), -f(FZ, U_BX, i, j, k), R(1, i - 1, j, k - 1), R(1, i, j, k - 1));
+        ez += upwind_corr(f(FX, U_RHO, i, j, k), f(FY, U_BX, i - 1, j, k), f(FY, U_BX, i, j, k), R(2, i - 1, j, k), R(2, i, j, k));
+        ez += upwind_corr(f(FX, U_RHO, i, j - 1, k), f(FY, U_BX, i - 1, j, k), f(FY, U_BX, i, j, k), R(2, i - 1, j - 1, k), R(2, i, j - 1, k));
+        ex += upwind_corr(f(FY, U_RHO, i, j, k), f(FZ, U_BY, i, j - 1, k), f(FZ, U_BY, i, j, k), R(0, i, j - 1, k), R(0, i, j, k));
+        ex += upwind_corr(f(FY, U_RHO, i, j, k - 1), f(FZ, U_BY, i, j - 1, k), f(FZ, U_BY, i, j, k), R(0, i, j - 1, k - 1), R(0, i, j, k - 1));
+        ez += upwind_corr(f(FY, U_RHO, i, j, k), -f(FX, U_BY, i, j - 1, k), -f(FX, U_BY, i, j, k), R(2, i, j - 1, k), R(2, i, j, k));
+        ez += upwind_corr(f(FY, U_RHO, i - 1, j, k), -f(FX, U_BY, i, j - 1, k), -f(FX, U_BY, i, j, k), R(2, i - 1, j - 1, k), R(2, i - 1, j, k));
+        ex += upwind_corr(f(FZ, U_RHO, i, j, k), -f(FY, U_BZ, i, j, k - 1), -f(FY, U_BZ, i, j, k), R(0, i, j, k - 1), R(0, i, j, k));
+        ex += upwind_corr(f(FZ, U_RHO, i, j - 1, k), -f(FY, U_BZ, i, j, k - 1), -f(FY, U_BZ, i, j, k), R(0, i, j - 1, k - 1), R(0, i, j - 1, k));
+        ey += upwind_corr(f(FZ, U_RHO, i, j, k), f(FX, U_BZ, i, j, k - 1), f(FX, U_BZ, i, j, k), R(1, i, j, k - 1), R(1, i, j, k));
+        ey += upwind_corr(f(FZ, U_RHO, i - 1, j, k), f(FX, U_BZ, i, j, k - 1), f(FX, U_BZ, i, j, k), R(1, i - 1, j, k - 1), R(1, i - 1, j, k));
+    }
+    const long e = fe.at(i, j, k);
+    Ex[e] = ex; Ey[e] = ey; Ez[e] = ez;
+}
+
+// Faraday (B_fields.cpp:21-66) fused with the clone: Bout = Bin everywhere,
+// minus dt*curl(E) on the faces of the reference's ranges (ghost width gg).
+template <int D>
+KB __global__ void k_faraday(const double* Ex, const double* Ey, const double* Ez, const double* Bix, const double* Biy,
+                             const double* Biz, double* Box_, double* Boy, double* Boz, Ext fe, int gg, double dt_dx, double dt_dy,
+                             double dt_dz) {
+    const long m = tid();
+    if (m >= fe.size) return;
+    double bx = Bix[m], by = Biy[m], bz = Biz[m];
+    // recover (i,j,k) of m
+    long r = m;
+    int i, j = 0, k = 0;
+    const int Ny = D >= 2 ? fe.n[1] + 2 * fe.g : 1, Nz = D >= 3 ? fe.n[2] + 2 * fe.g : 1;
+    if (D == 3) { k = (int)(r % Nz) - fe.g; r /= Nz; }
+    if (D >= 2) { j = (int)(r % Ny) - fe.g; r /= Ny; }
+    i = (int)r - fe.g;
+    const int nx = fe.n[0] - 1, ny = fe.n[1] - 1, nz = fe.n[2] - 1;  // B sizes are n+1
+    const bool ix = i >= -gg && i <= nx + gg, ixl = i >= -gg && i < nx + gg;
+    const bool jy = j >= -gg && j <= ny + gg, jyl = j >= -gg && j < ny + gg;
+    const bool kz = k >= -gg && k <= nz + gg, kzl = k >= -gg && k < nz + gg;
+    if (D == 2) {
+        if (ix && jyl) bx -= (Ez[m + fe.s[1]] - Ez[m]) * dt_dy;
+        if (ixl && jy) by -= -(Ez[m + fe.s[0]] - Ez[m]) * dt_dx;
+        if (ixl && jyl) bz -= (Ey[m + fe.s[0]] - Ey[m]) * dt_dx - (Ex[m + fe.s[1]] - Ex[m]) * dt_dy;
+    } else {
+        if (ix && jyl && kzl) bx -= (Ez[m + fe.s[1]] - Ez[m]) * dt_dy - (Ey[m + fe.s[2]] - Ey[m]) * dt_dz;
+        if (ixl && jy && kzl) by -= (Ex[m + fe.s[2]] - Ex[m]) * dt_dz - (Ez[m + fe.s[0]] - Ez[m]) * dt_dx;
+        if (ixl && jyl && kz) bz -= (Ey[m + fe.s[0]] - Ey[m]) * dt_dx - (Ex[m + fe.s[1]] - Ex[m]) * dt_dy;
+    }
+    Box_[m] = bx; Boy[m] = by; Boz[m] = bz;
+}
+
+// computeBodyField + shouldProtectThermal (B_fields.cpp:70-114, production
+// beta-gated form), applied to a freshly updated conservative state.
+__device__ __forceinline__ P<double> body_field(P<double> w, double cbx, double cby, double cbz, const Phys& ph) {
+    bool protect;
+    if (ph.ct_energy == 0) protect = false;
+    else if (ph.ct_energy == 1) protect = true;
+    else {
+        double B2 = (w.Bx * w.Bx + w.By * w.By + w.Bz * w.Bz);
+        protect = B2 == 0 ? false : (ph.eightpi * w.p / (w.Bx * w.Bx + w.By * w.By + w.Bz * w.Bz) <= ph.ct_beta);
+    }
+    if (!protect) {
+        U<double> u = cons<true>(w, ph);
+        u.Bx = cbx; u.By = cby; u.Bz = cbz;
+        return prim<true>(u, ph);
+    }
+    w.Bx = cbx; w.By = cby; w.Bz = cbz;
+    return w;
+}
+__device__ __forceinline__ void cell_B(const double* Bx, const double* By, const double* Bz, const Ext& fe, int D, int i, int j, int k,
+                                       double& x, double& y, double& z) {
+    const long m = fe.at(i, j, k);
+    x = (Bx[m] + Bx[m + fe.s[0]]) * 0.5;
+    y = (By[m] + By[m + fe.s[1]]) * 0.5;
+    z = D == 3 ? (Bz[m] + Bz[m + fe.s[2]]) * 0.5 : Bz[m];
+}
+
+// Half-step state for the CTU corrector (CTU.cpp:57-61, 154-158): applyFluxes
+// at dt/2 over cells [-1, n+1), body fields from the half-step faces, then
+// Ehalf = cross(B, v) over cells [-1, n].
+template <int D>
+KB __global__ void k_whalf(Soa W, Soa FX, Soa FY, Soa FZ, const double* Bhx, const double* Bhy, const double* Bhz, Ext ce,
+                           Ext fe, Box box, double hx, double hy, double hz, Phys ph, double* Ehx, double* Ehy, double* Ehz,
+                           unsigned long long* err) {
+    int i, j, k;
+    if (!box.point(tid(), i, j, k)) return;
+    const long m = ce.at(i, j, k);
+    P<double> w = load_p<double>(W, m, true);
+    U<double> u = cons<true>(w, ph);
+    u = uadd<true>(u, umul<true>(usub<true>(load_u(FX, m, true), load_u(FX, m + ce.s[0], true)), hx));
+    u = uadd<true>(u, umul<true>(usub<true>(load_u(FY, m, true), load_u(FY, m + ce.s[1], true)), hy));
+    if (D == 3) u = uadd<true>(u, umul<true>(usub<true>(load_u(FZ, m, true), load_u(FZ, m + ce.s[2], true)), hz));
+    if (!finite<true>(u)) report(err, ST_WHALF_NAN, (unsigned long long)m, E_NAN);
+    w = prim<true>(u, ph);
+    double cbx, cby, cbz;
+    cell_B(Bhx, Bhy, Bhz, fe, D, i, j, k, cbx, cby, cbz);
+    w = body_field(w, cbx, cby, cbz, ph);
+    double ex, ey, ez;
+    cross(w.Bx, w.By, w.Bz, w.vx, w.vy, w.vz, ex, ey, ez);
+    const long e = fe.at(i, j, k);
+    Ehx[e] = ex; Ehy[e] = ey; Ehz[e] = ez;
+}
+
+// ctu_sweep_MHD corrections (CTU.cpp:65-129 3D, 162-202 2D), in place.
+template <int D>
+KB __global__ void k_ctu_mhd(Soa W, const double* Bx, const double* By, const double* Bz, const double* Bhx, const double* Bhy,
+                             const double* Bhz, const double* E0x, const double* E0y, const double* E0z, Soa FX, Soa FY, Soa FZ,
+                             Soa XL, Soa XR, Soa YL, Soa YR, Soa ZL, Soa ZR, Ext ce, Ext fe, Box box, double dt_dx, double dt_dy,
+                             double dt_dz, Phys ph) {
+    int i, j, k;
+    if (!box.point(tid(), i, j, k)) return;
+    const long m = ce.at(i, j, k), q = fe.at(i, j, k);
+    const long fx = fe.s[0], fy = fe.s[1], fz = fe.s[2];
+    const double k8 = ph.k1_8pi;
+    const P<double> Wc = load_p<double>(W, m, true);
+    const double dBx = (Bx[q + fx] - Bx[q]) * dt_dx;
+    const double dBy = (By[q + fy] - By[q]) * dt_dy;
+    auto apply = [&](Soa& HL, Soa& HR, U<double> corr, int ax) {
+        U<double> uL = uadd<true>(cons<true>(load_p<double>(HL, m, true), ph), corr);
+        U<double> uR = uadd<true>(cons<true>(load_p<double>(HR, m, true), ph), corr);
+        if (ax == 0) { uL.Bx = Bhx[q]; uR.Bx = Bhx[q + fx]; }
+        else if (ax == 1) { uL.By = Bhy[q]; uR.By = Bhy[q + fy]; }
+        else { uL.Bz = Bhz[q]; uR.Bz = Bhz[q + fz]; }
+        store_p<double>(HL, m, prim<true>(uL, ph), true);
+        store_p<double>(HR, m, prim<true>(uR, ph), true);
+    };
+    auto fd = [&](const Soa& F, int ax) { return usub<true>(load_u(F, m, true), load_u(F, m + ce.s[ax], true)); };
+    if (D == 2) {
+        U<double> corr = umul<true>(umul<true>(fd(FY, 1), 0.5), dt_dy);
+        corr.Bx = corr.By = corr.Bz = 0;
+        double lim_z = minmod(-dBy, dBx);
+        corr.E += k8 * (Wc.Bz * Wc.vz * lim_z);
+        corr.Bz = 0.5 * dt_dy * (E0x[q + fy] - E0x[q]) + 0.5 * Wc.vz * lim_z;
+        apply(XL, XR, corr, 0);
+        corr = umul<true>(umul<true>(fd(FX, 0), 0.5), dt_dx);
+        corr.Bx = corr.By = corr.Bz = 0;
+        lim_z = minmod(-dBx, dBy);
+        corr.E += k8 * (Wc.Bz * Wc.vz * lim_z);
+        corr.Bz = -0.5 * dt_dx * (E0y[q + fx] - E0y[q]) + 0.5 * Wc.vz * lim_z;
+        apply(YL, YR, corr, 1);
+    } else {
+        const double dBz = (Bz[q + fz] - Bz[q]) * dt_dz;
+        // x half states
+        U<double> corr = umul<true>(uadd<true>(umul<true>(fd(FY, 1), dt_dy), umul<true>(fd(FZ, 2), dt_dz)), 0.5);
+        corr.Bx = corr.By = corr.Bz = 0;
+        double lim_y = minmod(-dBz, dBx), lim_z = minmod(-dBy, dBx);
+        corr.E += k8 * (Wc.By * Wc.vy * lim_y + Wc.Bz * Wc.vz * lim_z);
+        corr.By = -0.25 * dt_dz * (E0x[q + fz] - E0x[q] + E0x[q + fy + fz] - E0x[q + fy]) + 0.5 * Wc.vy * lim_y;
+        corr.Bz = 0.25 * dt_dy * (E0x[q + fy] - E0x[q] + E0x[q + fy + fz] - E0x[q + fz]) + 0.5 * Wc.vz * lim_z;
+        apply(XL, XR, corr, 0);
+        // y half states
+        corr = umul<true>(uadd<true>(umul<true>(fd(FX, 0), dt_dx), umul<true>(fd(FZ, 2), dt_dz)), 0.5);
+        corr.Bx = corr.By = corr.Bz = 0;
+        double lim_x = minmod(-dBz, dBy);
+        lim_z = minmod(-dBx, dBy);
+        corr.E += k8 * (Wc.Bx * Wc.vx * lim_x + Wc.Bz * Wc.vz * lim_z);
+        corr.Bx = 0.25 * dt_dz * (E0y[q + fz] - E0y[q] + E0y[q + fx + fz] - E0y[q + fx]) + 0.5 * Wc.vx * lim_x;
+        corr.Bz = -0.25 * dt_dx * (E0y[q + fx] - E0y[q] + E0y[q + fx + fz] - E0y[q + fz]) + 0.5 * Wc.vz * lim_z;
+        apply(YL, YR, corr, 1);
+        // z half states
+        corr = umul<true>(uadd<true>(umul<true>(fd(FX, 0), dt_dx), umul<true>(fd(FY, 1), dt_dy)), 0.5);
+        corr.Bx = corr.By = corr.Bz = 0;
+        lim_x = minmod(-dBy, dBz);
+        lim_y = minmod(-dBx, dBz);
+        corr.E += k8 * (Wc.Bx * Wc.vx * lim_x + Wc.By * Wc.vy * lim_y);
+        corr.Bx = -0.25 * dt_dy * (E0z[q + fy] - E0z[q] + E0z[q + fx + fy] - E0z[q + fx]) + 0.5 * Wc.vx * lim_x;
+        corr.By = 0.25 * dt_dx * (E0z[q + fx] - E0z[q] + E0z[q + fx + fy] - E0z[q + fy]) + 0.5 * Wc.vy * lim_y;
+        apply(ZL, ZR, corr, 2);
+    }
+}
+
+// ========================================================= apply + commit
+// applyFluxes (Unsplit.cpp:256-288) + computeBodyFields with the new faces +
+// the physicality check (Unsplit.cpp:94-99, 188-195), writing the complete
+// next state: interior cells updated, ghost cells copied (the reference
+// commits interior cells only, so the old ghosts remain).
+template <int D, bool MHD>
+KB __global__ void k_apply(Soa W, Soa Wn, Soa FX, Soa FY, Soa FZ, const double* Bnx, const double* Bny, const double* Bnz, Ext ce,
+                           Ext fe, double dt_dx, double dt_dy, double dt_dz, Phys ph, unsigned long long* err) {
+    const long m = tid();
+    if (m >= ce.size) return;
+    const int nf = MHD ? 8 : 5;
+    long r = m;
+    int i, j = 0, k = 0;
+    const int Ny = D >= 2 ? ce.n[1] + 2 * ce.g : 1, Nz = D >= 3 ? ce.n[2] + 2 * ce.g : 1;
+    if (D == 3) { k = (int)(r % Nz) - ce.g; r /= Nz; }
+    if (D >= 2) { j = (int)(r % Ny) - ce.g; r /= Ny; }
+    i = (int)r - ce.g;
+    const bool interior = i >= 0 && i < ce.n[0] && j >= 0 && j < ce.n[1] && k >= 0 && k < ce.n[2];
+    if (!interior) {
+        for (int f = 0; f < nf; f++) Wn(f, m) = W(f, m);
+        return;
+    }
+    P<double> w = load_p<double>(W, m, MHD);
+    U<double> u = cons<MHD>(w, ph);
+    u = uadd<MHD>(u, umul<MHD>(usub<MHD>(load_u(FX, m, MHD), load_u(FX, m + ce.s[0], MHD)), dt_dx));
+    if (D >= 2) u = uadd<MHD>(u, umul<MHD>(usub<MHD>(load_u(FY, m, MHD), load_u(FY, m + ce.s[1], MHD)), dt_dy));
+    if (D == 3) u = uadd<MHD>(u, umul<MHD>(usub<MHD>(load_u(FZ, m, MHD), load_u(FZ, m + ce.s[2], MHD)), dt_dz));
+    if (D >= 2 && !finite<MHD>(u)) report(err, ST_APPLY_NAN, (unsigned long long)m, E_NAN);
+    w = prim<MHD>(u, ph);
+    if (MHD && D >= 2) {
+        double cbx, cby, cbz;
+        cell_B(Bnx, Bny, Bnz, fe, D, i, j, k, cbx, cby, cbz);
+        w = body_field(w, cbx, cby, cbz, ph);
+    }
+    if (!physical<MHD>(w)) report(err, ST_UNPHYSICAL, (unsigned long long)m, E_UNPHYSICAL);
+    store_p<double>(Wn, m, w, MHD);
+}
+
+// 1D Godunov::sweep update (Splitting.cpp:67-95): cells [-g+1, n+g-2] take
+// the flux difference; cell n+g-2 then takes the sweep's trailing zero
+// update (a prim->cons->prim round trip).  Ghosts are committed too.
+template <bool MHD>
+KB __global__ void k_apply1(Soa W, Soa Wn, Soa F, Ext ce, double dt_dx, Phys ph, unsigned long long* err) {
+    const long m = tid();
+    if (m >= ce.size) return;
+    const int nf = MHD ? 8 : 5, g = ce.g, n = ce.n[0];
+    const int i = (int)m - g;
+    if (i < -g + 1 || i > n + g - 2) {
+        for (int f = 0; f < nf; f++) Wn(f, m) = W(f, m);
+        return;
+    }
+    P<double> w = load_p<double>(W, m, MHD);
+    U<double> u = uadd<MHD>(cons<MHD>(w, ph), umul<MHD>(usub<MHD>(load_u(F, m, MHD), load_u(F, m + 1, MHD)), dt_dx));
+    w = prim<MHD>(u, ph);
+    if (i == n + g - 2) {
+        U<double> fr = load_u(F, m + 1, MHD);
+        u = uadd<MHD>(cons<MHD>(w, ph), umul<MHD>(usub<MHD>(fr, fr), dt_dx));
+        w = prim<MHD>(u, ph);
+    }
+    if (i >= 0 && i < n && !physical<MHD>(w)) report(err, ST_UNPHYSICAL, (unsigned long long)m, E_UNPHYSICAL);
+    store_p<double>(Wn, m, w, MHD);
+}
+
+// AoS <-> SoA staging for upload/download in the reference's record layout.
+KB __global__ void k_aos_to_soa(const double* a, int stride, int nf, Soa s, long n) {
+    const long m = tid();
+    if (m >= n) return;
+    for (int f = 0; f < nf; f++) s(f, m) = f < stride ? a[m * stride + f] : 0.0;
+}
+KB __global__ void k_soa_to_aos(Soa s, int nf, double* a, int stride, long n) {
+    const long m = tid();
+    if (m >= n) return;
+    for (int f = 0; f < stride; f++) a[m * stride + f] = f < nf ? s(f, m) : 0.0;
+}
+KB __global__ void k_fill_u64(unsigned long long* p, unsigned long long v, int n) {
+    const int t = (int)tid();
+    if (t < n) p[t] = v;
+}
+
+// ============================================================== launchers
+static inline unsigned blocks(long n) { return (unsigned)((n + 255) / 256); }
+
+// initialize_B_fields (B_fields.cpp:117-149): cell B = face average on
+// every cell, pressure held.
+template <int D>
+KB __global__ void k_init_B(Soa W, const double* Bx, const double* By, const double* Bz, Ext ce, Ext fe) {
+    Box box = make_box(D, -ce.g, ce.n[0] + ce.g, -ce.g, ce.n[1] + ce.g, -ce.g, ce.n[2] + ce.g);
+    int i, j, k;
+    if (!box.point(tid(), i, j, k)) return;
+    double x, y, z;
+    cell_B(Bx, By, Bz, fe, D, i, j, k, x, y, z);
+    const long m = ce.at(i, j, k);
+    W(F_BX, m) = x; W(F_BY, m) = y; W(F_BZ, m) = z;
+}
+// Divergence monitor: max |div B| and max |B_cell| over interior cells.
+template <int D>
+KB __global__ void k_divb(const double* Bx, const double* By, const double* Bz, Ext ce, Ext fe, double dx, double dy, double dz,
+                          unsigned long long* out) {
+    Box box = make_box(D, 0, ce.n[0], 0, ce.n[1], 0, ce.n[2]);
+    int i, j, k;
+    double dv = 0, bm = 0;
+    if (box.point(tid(), i, j, k)) {
+        const long q = fe.at(i, j, k);
+        double d = (Bx[q + fe.s[0]] - Bx[q]) / dx + (By[q + fe.s[1]] - By[q]) / dy;
+        if (D == 3) d += (Bz[q + fe.s[2]] - Bz[q]) / dz;
+        dv = dabs(d);
+        double x, y, z;
+        cell_B(Bx, By, Bz, fe, D, i, j, k, x, y, z);
+        bm = dsqrt(x * x + y * y + z * z);
+    }
+    for (int off = 32; off > 0; off >>= 1) {
+        dv = dmax(dv, __shfl_xor(dv, off));
+        bm = dmax(bm, __shfl_xor(bm, off));
+    }
+    if ((threadIdx.x & 63) == 0) {
+        if (dv > 0) atomicMax(out, (unsigned long long)__double_as_longlong(dv));
+        if (bm > 0) atomicMax(out + 1, (unsigned long long)__double_as_longlong(bm));
+    }
+}
+void launch_init_B(int dim, Soa W, double* B, Ext ce, Ext fe, hipStream_t s) {
+    const long nf = fe.size;
+    if (dim == 2) hipLaunchKernelGGL(k_init_B<2>, dim3(blocks(ce.size)), dim3(256), 0, s, W, B, B + nf, B + 2 * nf, ce, fe);
+    else hipLaunchKernelGGL(k_init_B<3>, dim3(blocks(ce.size)), dim3(256), 0, s, W, B, B + nf, B + 2 * nf, ce, fe);
+}
+void launch_divb(int dim, double* B, Ext ce, Ext fe, const double d[3], unsigned long long* out, hipStream_t s) {
+    const long nf = fe.size;
+    Box b = make_box(dim, 0, ce.n[0], 0, ce.n[1], 0, ce.n[2]);
+    if (dim == 2) hipLaunchKernelGGL(k_divb<2>, dim3(blocks(b.count())), dim3(256), 0, s, B, B + nf, B + 2 * nf, ce, fe, d[0], d[1], d[2], out);
+    else hipLaunchKernelGGL(k_divb<3>, dim3(blocks(b.count())), dim3(256), 0, s, B, B + nf, B + 2 * nf, ce, fe, d[0], d[1], d[2], out);
+}
+
+void launch_bc(const BCArgs& a, bool faces, hipStream_t s) {
+    long n = (long)(a.hi[0] - a.lo[0]) * (a.hi[1] - a.lo[1]);
+    if (n <= 0) return;
+    if (faces) hipLaunchKernelGGL(k_bc_faces, dim3(blocks(n)), dim3(256), 0, s, a);
+    else hipLaunchKernelGGL(k_bc_cells, dim3(blocks(n)), dim3(256), 0, s, a);
+}
+
+void launch_fill_u64(unsigned long long* p, unsigned long long v, int n, hipStream_t s) {
+    hipLaunchKernelGGL(k_fill_u64, dim3(blocks(n)), dim3(256), 0, s, p, v, n);
+}
+void launch_aos_to_soa(const double* a, int stride, int nf, Soa soa, long n, hipStream_t s) {
+    hipLaunchKernelGGL(k_aos_to_soa, dim3(blocks(n)), dim3(256), 0, s, a, stride, nf, soa, n);
+}
+void launch_soa_to_aos(Soa soa, int nf, double* a, int stride, long n, hipStream_t s) {
+    hipLaunchKernelGGL(k_soa_to_aos, dim3(blocks(n)), dim3(256), 0, s, soa, nf, a, stride, n);
+}
+
+template <int D, bool MHD>
+static void cfl_t(const StepCtx& c, unsigned long long* out, hipStream_t s) {
+    const int n1 = c.ce.g < 1 ? c.ce.g : 1;
+    Box b = make_box(D, -n1, c.ce.n[0] + n1, -n1, c.ce.n[1] + n1, -n1, c.ce.n[2] + n1);
+    hipLaunchKernelGGL((k_cfl<D, MHD>), dim3(blocks(b.count())), dim3(256), 0, s, c.W, c.ce, b, c.ph, c.cfl_mode, c.d[0], c.d[1],
+                       c.d[2], out);
+}
+void launch_cfl(const StepCtx& c, unsigned long long* out, hipStream_t s) {
+    if (c.dim == 1) c.mhd ? cfl_t<1, true>(c, out, s) : cfl_t<1, false>(c, out, s);
+    else if (c.dim == 2) c.mhd ? cfl_t<2, true>(c, out, s) : cfl_t<2, false>(c, out, s);
+    else c.mhd ? cfl_t<3, true>(c, out, s) : cfl_t<3, false>(c, out, s);
+}
+
+template <int D, bool MHD, class R, int AX>
+static void flux_t(const StepCtx& c, const Soa& HL, const Soa& HR, const Soa& F, Box b, double dt_d, int stage, hipStream_t s) {
+    if (b.count() <= 0) return;
+    hipLaunchKernelGGL((k_flux<D, MHD, R, AX>), dim3(blocks(b.count())), dim3(256), 0, s, HL, HR, F, c.ce, b, dt_d, c.ph, c.err, stage);
+}
+template <int D, bool MHD, class R>
+static void flux_ax(const StepCtx& c, int ax, const Soa& HL, const Soa& HR, const Soa& F, Box b, double dt_d, int stage, hipStream_t s) {
+    if (ax == 0) flux_t<D, MHD, R, 0>(c, HL, HR, F, b, dt_d, stage, s);
+    else if (ax == 1) { if constexpr (D >= 2) flux_t<D, MHD, R, 1>(c, HL, HR, F, b, dt_d, stage, s); }
+    else { if constexpr (D >= 3) flux_t<D, MHD, R, 2>(c, HL, HR, F, b, dt_d, stage, s); }
+}
+
+// Face-range boxes of computeFlux_* for axis `ax`: [lo, hi] inclusive along
+// ax, [lo, hi) elsewhere; `w` is the transverse widening.
+static Box face_box(int D, const Ext& ce, int ax, int along_lo, int along_hi_incl, int tlo, int thi_extra) {
+    int lo[3], hi[3];
+    for (int d = 0; d < 3; d++) {
+        if (d == ax) { lo[d] = along_lo; hi[d] = ce.n[d] + along_hi_incl + 1; }
+        else { lo[d] = tlo; hi[d] = ce.n[d] + thi_extra; }
+    }
+    return make_box(D, lo[0], hi[0], lo[1], hi[1], lo[2], hi[2]);
+}
+
+template <int D, bool MHD, class R>
+static void step_t(const StepCtx& c, hipStream_t s) {
+    const Ext& ce = c.ce;
+    const double dt_d[3] = {c.dt / c.d[0], c.dt / c.d[1], c.dt / c.d[2]};
+    const int g = ce.g;
+    // half states
+    {
+        Box all = make_box(D, -g, ce.n[0] + g, -g, ce.n[1] + g, -g, ce.n[2] + g);
+        hipLaunchKernelGGL((k_half<D, MHD, R>), dim3(blocks(all.count())), dim3(256), 0, s, c.W, c.B[0], c.B[1], c.B[2], ce, c.fe,
+                           c.HS[0], c.HS[1], c.HS[2], c.HS[3], c.HS[4], c.HS[5], dt_d[0], dt_d[1], dt_d[2], c.ph);
+    }
+    if (D == 1) {
+        Box fb = make_box(1, -g + 1, ce.n[0] + g, 0, 1, 0, 1);
+        flux_ax<D, MHD, R>(c, 0, c.HS[0], c.HS[1], c.F[0], fb, dt_d[0], ST_FLUX_X, s);
+        if constexpr (D == 1)
+            hipLaunchKernelGGL((k_apply1<MHD>), dim3(blocks(ce.size)), dim3(256), 0, s, c.W, c.Wn, c.F[0], ce, dt_d[0], c.ph, c.err);
+        return;
+    }
+    if (c.ctu) {
+        if constexpr (MHD) {
+            // ctu_sweep_MHD predictor fluxes on [-1, n+1] x [-2, n+2)
+            for (int ax = 0; ax < D; ax++)
+                flux_ax<D, MHD, R>(c, ax, c.HS[2 * ax], c.HS[2 * ax + 1], c.F[ax], face_box(D, ce, ax, -1, 1, -2, 2), dt_d[ax],
+                                   ST_PRED_X + ax, s);
+            ERef none{nullptr, nullptr, nullptr};
+            Box eb = make_box(D, -1, ce.n[0] + 2, -1, ce.n[1] + 2, -1, ce.n[2] + 2);
+            if constexpr (D == 2)
+                hipLaunchKernelGGL((k_emf2<true>), dim3(blocks(eb.count())), dim3(256), 0, s, c.F[0], c.F[1], c.W, none, ce, c.fe, eb, 1,
+                                   c.E0[0], c.E0[1], c.E0[2]);
+            else
+                hipLaunchKernelGGL((k_emf3<true>), dim3(blocks(eb.count())), dim3(256), 0, s, c.F[0], c.F[1], c.F[2], c.W, none, ce, c.fe,
+                                   eb, 1, c.E0[0], c.E0[1], c.E0[2]);
+            hipLaunchKernelGGL((k_faraday<D>), dim3(blocks(c.fe.size)), dim3(256), 0, s, c.E0[0], c.E0[1], c.E0[2], c.B[0], c.B[1],
+                               c.B[2], c.Bh[0], c.Bh[1], c.Bh[2], c.fe, 1, 0.5 * dt_d[0], 0.5 * dt_d[1], 0.5 * dt_d[2]);
+            Box cb = make_box(D, -1, ce.n[0] + 1, -1, ce.n[1] + 1, -1, ce.n[2] + 1);
+            hipLaunchKernelGGL((k_whalf<D>), dim3(blocks(cb.count())), dim3(256), 0, s, c.W, c.F[0], c.F[1], c.F[2], c.Bh[0], c.Bh[1],
+                               c.Bh[2], ce, c.fe, cb, 0.5 * dt_d[0], 0.5 * dt_d[1], 0.5 * dt_d[2], c.ph, c.Eh[0], c.Eh[1], c.Eh[2],
+                               c.err);
+            hipLaunchKernelGGL((k_ctu_mhd<D>), dim3(blocks(cb.count())), dim3(256), 0, s, c.W, c.B[0], c.B[1], c.B[2], c.Bh[0], c.Bh[1],
+                               c.Bh[2], c.E0[0], c.E0[1], c.E0[2], c.F[0], c.F[1], c.F[2], c.HS[0], c.HS[1], c.HS[2], c.HS[3],
+                               c.HS[4], c.HS[5], ce, c.fe, cb, dt_d[0], dt_d[1], dt_d[2], c.ph);
+        } else if constexpr (D == 2) {
+            // ctu_sweep_hydro 2D (CTU.cpp:206-218)
+            flux_ax<D, MHD, R>(c, 0, c.HS[0], c.HS[1], c.F[0], face_box(D, ce, 0, 0, 0, -1, 1), dt_d[0], ST_PRED_X, s);
+            flux_ax<D, MHD, R>(c, 1, c.HS[2], c.HS[3], c.F[1], face_box(D, ce, 1, 0, 0, -1, 1), dt_d[1], ST_PRED_Y, s);
+            Box bx = make_box(2, -1, ce.n[0] + 1, 0, ce.n[1]);
+            hipLaunchKernelGGL((k_correct<D, R>), dim3(blocks(bx.count())), dim3(256), 0, s, c.HS[0], c.HS[1], c.F[1], ce, bx, 1,
+                               0.5 * dt_d[1], c.ph);
+            Box by = make_box(2, 0, ce.n[0], -1, ce.n[1] + 1);
+            hipLaunchKernelGGL((k_correct<D, R>), dim3(blocks(by.count())), dim3(256), 0, s, c.HS[2], c.HS[3], c.F[0], ce, by, 0,
+                               0.5 * dt_d[0], c.ph);
+        } else {
+            ctu_hydro3(c, s);
+        }
+    }
+    // final fluxes
+    if (MHD) {
+        for (int ax = 0; ax < D; ax++)
+            flux_ax<D, MHD, R>(c, ax, c.HS[2 * ax], c.HS[2 * ax + 1], c.F[ax], face_box(D, ce, ax, 0, 0, -1, 1), dt_d[ax],
+                               ST_FLUX_X + ax, s);
+    } else {
+        for (int ax = 0; ax < D; ax++)
+            flux_ax<D, MHD, R>(c, ax, c.HS[2 * ax], c.HS[2 * ax + 1], c.F[ax], face_box(D, ce, ax, 0, 0, 0, 0), dt_d[ax],
+                               ST_FLUX_X + ax, s);
+    }
+    if constexpr (MHD) {
+        Box eb = make_box(D, 0, ce.n[0] + 1, 0, ce.n[1] + 1, 0, ce.n[2] + 1);
+        ERef eh{c.Eh[0], c.Eh[1], c.Eh[2]};
+        if constexpr (D == 2)
+            hipLaunchKernelGGL((k_emf2<false>), dim3(blocks(eb.count())), dim3(256), 0, s, c.F[0], c.F[1], c.W, eh, ce, c.fe, eb, c.ctu,
+                               c.E0[0], c.E0[1], c.E0[2]);
+        else
+            hipLaunchKernelGGL((k_emf3<false>), dim3(blocks(eb.count())), dim3(256), 0, s, c.F[0], c.F[1], c.F[2], c.W, eh, ce, c.fe, eb,
+                               c.ctu, c.E0[0], c.E0[1], c.E0[2]);
+        hipLaunchKernelGGL((k_faraday<D>), dim3(blocks(c.fe.size)), dim3(256), 0, s, c.E0[0], c.E0[1], c.E0[2], c.B[0], c.B[1], c.B[2],
+                           c.Bn[0], c.Bn[1], c.Bn[2], c.fe, 0, dt_d[0], dt_d[1], dt_d[2]);
+    }
+    hipLaunchKernelGGL((k_apply<D, MHD>), dim3(blocks(ce.size)), dim3(256), 0, s, c.W, c.Wn, c.F[0], c.F[1], c.F[2], c.Bn[0], c.Bn[1],
+                       c.Bn[2], ce, c.fe, dt_d[0], dt_d[1], dt_d[2], c.ph, c.err);
+}
+
+// 3D hydro CTU, 12 solves (CTU.cpp:222-261): six corner-flux passes with
+// dt/3 transverse corrections, then the state corrections.
+void ctu_hydro3(const StepCtx& c, hipStream_t s) {
+    const Ext& ce = c.ce;
+    const double dx = c.dt / c.d[0], dy = c.dt / c.d[1], dz = c.dt / c.d[2];
+    const Soa &xL = c.HS[0], &xR = c.HS[1], &yL = c.HS[2], &yR = c.HS[3], &zL = c.HS[4], &zR = c.HS[5];
+    const Soa &FX = c.F[0], &FY = c.F[1], &FZ = c.F[2];
+    const Soa *T = c.T;  // five scratch flux arrays
+    const int nx = ce.n[0], ny = ce.n[1], nz = ce.n[2];
+    flux_t<3, false, double, 0>(c, xL, xR, FX, make_box(3, 0, nx + 1, -1, ny + 1, -1, nz + 1), dx, ST_PRED_X, s);
+    flux_t<3, false, double, 1>(c, yL, yR, FY, make_box(3, -1, nx + 1, 0, ny + 1, -1, nz + 1), dy, ST_PRED_Y, s);
+    flux_t<3, false, double, 2>(c, zL, zR, FZ, make_box(3, -1, nx + 1, -1, ny + 1, 0, nz + 1), dz, ST_PRED_Z, s);
+    auto ctu = [&](int ax, const Soa& L, const Soa& Rr, const Soa& Tr, int tax, const Soa& out, double dd, double dtt, int stage) {
+        int isX = tax == 0, isY = tax == 1, isZ = tax == 2;
+        Box b;
+        if (ax == 0) b = make_box(3, 0, nx + 1, -1 + isY, ny + 1 - isY, -1 + isZ, nz + 1 - isZ);
+        else if (ax == 1) b = make_box(3, -1 + isX, nx + 1 - isX, 0, ny + 1, -1 + isZ, nz + 1 - isZ);
+        else b = make_box(3, -1 + isX, nx + 1 - isX, -1 + isY, ny + 1 - isY, 0, nz + 1);
+        if (ax == 0)
+            hipLaunchKernelGGL((k_ctu_flux<double, 0>), dim3(blocks(b.count())), dim3(256), 0, s, L, Rr, Tr, out, ce, b, tax, dd, dtt, c.ph, c.err, stage);
+        else if (ax == 1)
+            hipLaunchKernelGGL((k_ctu_flux<double, 1>), dim3(blocks(b.count())), dim3(256), 0, s, L, Rr, Tr, out, ce, b, tax, dd, dtt, c.ph, c.err, stage);
+        else
+            hipLaunchKernelGGL((k_ctu_flux<double, 2>), dim3(blocks(b.count())), dim3(256), 0, s, L, Rr, Tr, out, ce, b, tax, dd, dtt, c.ph, c.err, stage);
+    };
+    auto corr = [&](const Soa& L, const Soa& Rr, const Soa& F, double dd, int sdim, int fdim) {
+        int lo[3] = {0, 0, 0}, hi[3] = {nx, ny, nz};
+        lo[sdim] = -1; hi[sdim] += 1;
+        Box b = make_box(3, lo[0], hi[0], lo[1], hi[1], lo[2], hi[2]);
+        hipLaunchKernelGGL((k_correct<3, double>), dim3(blocks(b.count())), dim3(256), 0, s, L, Rr, F, ce, b, fdim, dd, c.ph);
+    };
+    const Soa &FXz = T[0], &FYz = T[1], &FXy = T[2], &FZy = T[3], &FYx = T[4];
+    ctu(0, xL, xR, FZ, 2, FXz, dx, dz / 3.0, ST_CTU_F0 + 0);
+    ctu(1, yL, yR, FZ, 2, FYz, dy, dz / 3.0, ST_CTU_F0 + 1);
+    ctu(0, xL, xR, FY, 1, FXy, dx, dy / 3.0, ST_CTU_F0 + 2);
+    ctu(2, zL, zR, FY, 1, FZy, dz, dy / 3.0, ST_CTU_F0 + 3);
+    corr(xL, xR, FYz, 0.5 * dy, 0, 1);
+    corr(xL, xR, FZy, 0.5 * dz, 0, 2);
+    ctu(1, yL, yR, FX, 0, FYx, dy, dx / 3.0, ST_CTU_F0 + 4);
+    const Soa& FZx = FZ;  // FZ is no longer needed
+    ctu(2, zL, zR, FX, 0, FZx, dz, dx / 3.0, ST_CTU_F0 + 5);
+    corr(yL, yR, FXz, 0.5 * dx, 1, 0);
+    corr(yL, yR, FZx, 0.5 * dz, 1, 2);
+    corr(zL, zR, FXy, 0.5 * dx, 2, 0);
+    corr(zL, zR, FYx, 0.5 * dy, 2, 1);
+}
+
+void launch_step(const StepCtx& c, hipStream_t s) {
+    const bool f32 = c.mixed;
+    if (c.dim == 1) {
+        if (c.mhd) f32 ? step_t<1, true, float>(c, s) : step_t<1, true, double>(c, s);
+        else f32 ? step_t<1, false, float>(c, s) : step_t<1, false, double>(c, s);
+    } else if (c.dim == 2) {
+        if (c.mhd) f32 ? step_t<2, true, float>(c, s) : step_t<2, true, double>(c, s);
+        else f32 ? step_t<2, false, float>(c, s) : step_t<2, false, double>(c, s);
+    } else {
+        if (c.mhd) f32 ? step_t<3, true, float>(c, s) : step_t<3, true, double>(c, s);
+        else f32 ? step_t<3, false, float>(c, s) : step_t<3, false, double>(c, s);
+    }
+}
+
+}  // namespace dg

@@ -1,0 +1,137 @@
+/* dragon_gpu.h -- C ABI of the MI355X unsplit Godunov step.
+ *
+ * A drop-in for the reference's Grid1D/2D/3D step path: the grid state lives
+ * in HBM, and boundaries, CFL and the unsplit step run as HIP kernels on one
+ * stream per handle.  Plain pointers and sizes only; no exceptions cross this
+ * boundary.  Every function returns a DG_* status code.
+ *
+ * Reference citations are paths under the reference tree (DRAGON/...).
+ *
+ * Layouts (host and device buffers alike):
+ *   cell state   reference ExtendedArray order (Hydro/ExtendedArray/
+ *                ExtendedArray.hpp:75,125: last index fastest, ghosts
+ *                included), `stride` doubles per cell in PrimitiveState order
+ *                rho, p, vx, vy, vz[, Bx, By, Bz] (FluidElement.hpp:34-43).
+ *   face B       same order over the (n+1)-sized face index space, 3 doubles
+ *                (Bx, By, Bz) per face (Hydro/Grid.hpp:65,107).  2D: .z is
+ *                the cell-centred Bz, as in the reference.
+ *   SoA (device) field-major: field f of element m at base[f*count + m].
+ *
+ * Calls on one handle are serialised by the caller; handles are independent.
+ */
+#ifndef DRAGON_GPU_H
+#define DRAGON_GPU_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* status codes */
+enum {
+    DG_OK = 0,
+    DG_ERR_UNPHYSICAL = 1,        /* "Unphysical state would be produced at (i,j)"  Hydro/Godunov/Unsplit.cpp:97 */
+    DG_ERR_NAN = 2,               /* "NaN state would be produced at (i,j)"          Unsplit.cpp:267 */
+    DG_ERR_RIEMANN_INPUT = 3,     /* "Riemann Solution is unphysical because inputs were not physical"  Riemann/Flux.cpp:94 */
+    DG_ERR_RIEMANN_SOLUTION = 4,  /* "Riemann Solution to physical inputs is unphysical"                Riemann/Flux.cpp:95 */
+    DG_ERR_CFL = 5,               /* "CFL timestep failed: max signal speed is zero"  Hydro/CFL/CFL.cpp:127 */
+    DG_ERR_DT_UNDERFLOW = 6,      /* "Timestep has fallen below minimum."  CFL.cpp:230-233 (reference exits) */
+    DG_ERR_ARG = 7,
+    DG_ERR_HIP = 8
+};
+
+/* Config.h switches (Config.h:27-110) and the grid geometry (Grid.hpp:46,70,112). */
+enum { DG_EXACT = 0, DG_HLL = 1, DG_HLLC = 2, DG_HLLD = 3, DG_HLLE = 4, DG_HLLX = 5 };
+enum { DG_MINMOD = 0, DG_MC = 1, DG_VANLEER = 2, DG_SUPERBEE = 3, DG_VANALBADA = 4 };
+enum { DG_CT_TOTAL_E = 0, DG_CT_THERMAL = 1, DG_CT_BETA_GATED = 2 };
+enum { DG_CFL_MAX = 0, DG_CFL_ADD = 1 };   /* values > 1: power mean, as Config.h:70-77 */
+enum { DG_FP64 = 0, DG_MIXED = 1 };        /* precision of reconstruction + Riemann solves */
+
+typedef struct dg_desc {
+    int dim;                 /* 1, 2, 3 */
+    int nx, ny, nz;          /* interior cells */
+    double dx, dy, dz;
+    int ghosts;              /* requested; raised as validGhosts (Splitting.cpp:19-27) */
+    int mhd;                 /* #define MHD */
+    double gamma;            /* Constants.h:18 */
+    int solver;              /* RIEMANN_DEFAULT */
+    int limiter;             /* MUSCL_DEFAULT_LIMITER */
+    int muscl;               /* MUSCL_Hancock */
+    int ctu;                 /* CTU */
+    int ct_energy;           /* CT_ENERGY_CONSV */
+    double ct_beta;          /* CT_Energy_Beta */
+    int cfl_mode;            /* CFL_CALCULATION */
+    double cfl_coeff;        /* CFL_coeff */
+    int verify_fallback;     /* RIEMANN_VERIFY_FALLBACK: -1 = reference default (on iff hydro) */
+    int fallback_try_hlle;   /* RIEMANN_FALLBACK_TRY_HLLE */
+    double fallback_param;   /* Riemann_ExactFallback_Parameter */
+    int hlld_safety;         /* HLLD_PHYSICAL_SAFETY */
+    double timestep_tol;     /* Timestep_Tolerance */
+    int precision;           /* DG_FP64 (drop-in) or DG_MIXED */
+} dg_desc;
+
+/* Boundary kinds (Boundary/Boundary.hpp:160-229) and face bits (:139-149). */
+enum { DG_BC_OUTFLOW = 0, DG_BC_PERIODIC = 1, DG_BC_REFLECTIVE = 2, DG_BC_FIXED = 3, DG_BC_IGNORE = 4 };
+enum { DG_XN = 1, DG_XP = 2, DG_YN = 4, DG_YP = 8, DG_ZN = 16, DG_ZP = 32 };
+typedef struct dg_bc {
+    int kind;
+    int faces;               /* bit mask; periodic is widened to whole dimensions */
+    int corners;             /* corner_ghosts */
+    int gated;               /* Outflow::Gated */
+    double state[8];         /* Fixed: PrimitiveState rho, p, vx, vy, vz, Bx, By, Bz */
+} dg_bc;
+
+typedef struct dg_status {
+    int code;                /* DG_* */
+    int i, j, k;             /* cell of UNPHYSICAL / NAN */
+    char msg[192];           /* the reference's exception text */
+} dg_status;
+
+typedef struct dg_stats {
+    long steps;              /* accepted steps */
+    long failures;           /* rejected attempts (dt halvings) */
+    double t_advanced;
+} dg_stats;
+
+typedef struct dg_grid dg_grid;
+
+/* new GridND(n..., d..., ghosts) (Splitting.cpp:229-251) on the device. */
+int dg_create(const dg_desc* desc, dg_grid** out);
+int dg_destroy(dg_grid* g);
+/* getGhosts() (Grid.hpp:54,79,121) */
+int dg_ghosts(const dg_grid* g);
+/* element counts of the cell and face arrays, ghosts included */
+long dg_cell_count(const dg_grid* g);
+long dg_face_count(const dg_grid* g);
+
+/* Host state in/out: grid[i,j,k] and _B() of the reference (Grid.hpp:52-53,77-83,119-125).
+ * stride = doubles per cell record (5 hydro, 8 MHD; more is allowed). B may be NULL. */
+int dg_upload(dg_grid* g, const double* w, int stride, const double* B);
+int dg_download(dg_grid* g, double* w, int stride, double* B);
+/* Device-resident state in/out, SoA (8 cell fields, 3 face fields); no host copy. */
+int dg_upload_device(dg_grid* g, const double* w_soa, const double* B_soa);
+int dg_download_device(dg_grid* g, double* w_soa, double* B_soa);
+
+/* Grid::boundary = BoundaryList(...) (Boundary/BoundaryList.hpp:22-128): ordered list;
+ * uncovered faces get the implicit outflow first. */
+int dg_set_boundary(dg_grid* g, const dg_bc* list, int n);
+/* boundary.apply(grid) (BoundaryList.hpp:84-93) */
+int dg_apply_boundary(dg_grid* g);
+/* CFL::cfl_time(grid) (Hydro/CFL/CFL.cpp:113-162) */
+int dg_cfl_dt(dg_grid* g, double* dt, dg_status* st);
+/* GridND::unsplit_step(dt) (Unsplit.cpp:28-206, Splitting.cpp:100-115): no commit on failure. */
+int dg_step(dg_grid* g, double dt, dg_status* st);
+/* Grid::advance_unsplit(dt, check_cfl) (CFL.cpp:213-238); max_steps < 0: no limit. */
+int dg_advance(dg_grid* g, double dt, int check_cfl, long max_steps, dg_stats* stats, dg_status* st);
+/* Grid2D/3D::initialize_B_fields() (MHD/B_fields.cpp:117-149) */
+int dg_init_B(dg_grid* g);
+/* Error text of the last failing call on this handle. */
+const char* dg_last_error(const dg_grid* g);
+/* HIP stream the handle's kernels run on (as hipStream_t). */
+void* dg_stream(dg_grid* g);
+/* Diagnostics: max over interior cells of |div B| * min(dx) / max|B_face| (0 for hydro / 1D). */
+int dg_divb(dg_grid* g, double* out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
