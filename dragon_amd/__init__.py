@@ -104,6 +104,7 @@ def lib():
             "dg_last_error": (C.c_char_p, [vp]),
             "dg_stream": (vp, [vp]),
             "dg_divb": (C.c_int, [vp, _dp]),
+            "dg_timing": (C.c_int, [vp, C.c_int, _dp, C.POINTER(C.c_long)]),
         }
         for n, (r, a) in sig.items():
             f = getattr(L, n)
@@ -395,6 +396,14 @@ class Grid:
     def initialize_B_fields(self):
         self._push_bc()
         self._chk(lib().dg_init_B(self._h))
+
+    def timing(self, enable: int = -1):
+        """Start (1) / stop (0) / read (-1) live HIP-event timing of the step
+        kernels; returns (summed step ms, timed steps) before the change."""
+        ms = np.zeros(1)
+        n = C.c_long(0)
+        self._chk(lib().dg_timing(self._h, int(enable), ms.ctypes.data_as(_dp), C.byref(n)))
+        return float(ms[0]), int(n.value)
 
     def divb(self) -> float:
         out = np.zeros(1)

@@ -10,6 +10,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <string>
 #include <vector>
@@ -28,11 +29,17 @@ struct dg_grid {
     std::vector<dg_bc> bcs;
     double *W = nullptr, *Wn = nullptr;
     double *B = nullptr, *Bn = nullptr, *Bh = nullptr;
-    double *HS = nullptr, *F = nullptr, *T = nullptr;
+    double *HS = nullptr, *F = nullptr, *Fp = nullptr, *T = nullptr;
+    bool fused = true;
     double *E0 = nullptr, *Eh = nullptr;
     unsigned long long* dev_words = nullptr;   // [0] failure key, [1] CFL max bits, [2..3] diagnostics
     unsigned long long* host_words = nullptr;  // pinned mirror
     std::string err;
+    // optional live timing of the step kernels with HIP events
+    bool timing = false;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    double step_ms = 0;
+    long timed_steps = 0;
 };
 
 static const unsigned long long NO_ERR = ~0ull;
@@ -79,6 +86,7 @@ static StepCtx make_ctx(dg_grid* g, double dt) {
     StepCtx c{};
     c.dim = g->d.dim; c.mhd = g->d.mhd != 0; c.mixed = g->d.precision == DG_MIXED;
     c.ctu = g->d.ctu; c.cfl_mode = g->d.cfl_mode;
+    c.fused = g->fused;
     c.ce = g->ce; c.fe = g->fe; c.ph = g->ph;
     c.d[0] = g->d.dx; c.d[1] = g->d.dim >= 2 ? g->d.dy : 1.0; c.d[2] = g->d.dim >= 3 ? g->d.dz : 1.0;
     c.dt = dt;
@@ -91,8 +99,11 @@ static StepCtx make_ctx(dg_grid* g, double dt) {
         c.E0[q] = g->E0 ? g->E0 + q * nfc : nullptr;
         c.Eh[q] = g->Eh ? g->Eh + q * nfc : nullptr;
     }
-    for (int q = 0; q < 2 * g->d.dim; q++) c.HS[q] = soa(g->HS + (long)q * g->nf * nc, nc);
+    if (g->HS)
+        for (int q = 0; q < 2 * g->d.dim; q++) c.HS[q] = soa(g->HS + (long)q * g->nf * nc, nc);
     for (int q = 0; q < g->d.dim; q++) c.F[q] = soa(g->F + (long)q * g->nf * nc, nc);
+    if (g->Fp)
+        for (int q = 0; q < g->d.dim; q++) c.Fp[q] = soa(g->Fp + (long)q * g->nf * nc, nc);
     if (g->T)
         for (int q = 0; q < 5; q++) c.T[q] = soa(g->T + (long)q * g->nf * nc, nc);
     c.err = g->dev_words;
@@ -129,7 +140,13 @@ int dg_create(const dg_desc* desc, dg_grid** out) {
     };
     bool ok = hipStreamCreateWithFlags(&g->s, hipStreamNonBlocking) == hipSuccess;
     ok = ok && alloc(&g->W, 8 * nc) && alloc(&g->Wn, 8 * nc);
-    ok = ok && alloc(&g->HS, 2L * d.dim * g->nf * nc) && alloc(&g->F, (long)d.dim * g->nf * nc);
+    // fused sweeps (MHD 2D/3D, hydro 2D) need predictor + corrector fluxes;
+    // 1D and 3D hydro keep the reference's half-state arrays
+    const bool fusable = (d.mhd && d.dim >= 2) || (!d.mhd && d.dim == 2);
+    g->fused = fusable && getenv("DG_UNFUSED") == nullptr;
+    ok = ok && alloc(&g->F, (long)d.dim * g->nf * nc);
+    if (g->fused) ok = ok && (!d.ctu || alloc(&g->Fp, (long)d.dim * g->nf * nc));
+    else ok = ok && alloc(&g->HS, 2L * d.dim * g->nf * nc);
     if (!d.mhd && d.dim == 3 && d.ctu) ok = ok && alloc(&g->T, 5L * g->nf * nc);
     if (faces) ok = ok && alloc(&g->B, 3 * nfc) && alloc(&g->Bn, 3 * nfc) && alloc(&g->Bh, 3 * nfc) && alloc(&g->E0, 3 * nfc) &&
                     alloc(&g->Eh, 3 * nfc);
@@ -158,11 +175,13 @@ int dg_destroy(dg_grid* g) {
     if (!g) return DG_OK;
     int rc = DG_OK;
     if (g->s && hipStreamSynchronize(g->s) != hipSuccess) rc = DG_ERR_HIP;
-    double* bufs[] = {g->W, g->Wn, g->B, g->Bn, g->Bh, g->HS, g->F, g->T, g->E0, g->Eh};
+    double* bufs[] = {g->W, g->Wn, g->B, g->Bn, g->Bh, g->HS, g->F, g->Fp, g->T, g->E0, g->Eh};
     for (double* p : bufs)
         if (p && hipFree(p) != hipSuccess) rc = DG_ERR_HIP;
     if (g->dev_words && hipFree(g->dev_words) != hipSuccess) rc = DG_ERR_HIP;
     if (g->host_words && hipHostFree(g->host_words) != hipSuccess) rc = DG_ERR_HIP;
+    if (g->ev0 && hipEventDestroy(g->ev0) != hipSuccess) rc = DG_ERR_HIP;
+    if (g->ev1 && hipEventDestroy(g->ev1) != hipSuccess) rc = DG_ERR_HIP;
     if (g->s && hipStreamDestroy(g->s) != hipSuccess) rc = DG_ERR_HIP;
     delete g;
     return rc;
@@ -348,10 +367,18 @@ int dg_step(dg_grid* g, double dt, dg_status* st) {
     if (!g) return DG_ERR_ARG;
     launch_fill_u64(g->dev_words, NO_ERR, 1, g->s);
     StepCtx c = make_ctx(g, dt);
+    if (g->timing) HIPCHK(hipEventRecord(g->ev0, g->s));
     launch_step(c, g->s);
     HIPCHK(hipGetLastError());
+    if (g->timing) HIPCHK(hipEventRecord(g->ev1, g->s));
     HIPCHK(hipMemcpyAsync(g->host_words, g->dev_words, sizeof(unsigned long long), hipMemcpyDeviceToHost, g->s));
     HIPCHK(hipStreamSynchronize(g->s));
+    if (g->timing) {
+        float ms = 0;
+        HIPCHK(hipEventElapsedTime(&ms, g->ev0, g->ev1));
+        g->step_ms += ms;
+        g->timed_steps++;
+    }
     const unsigned long long key = g->host_words[0];
     if (key != NO_ERR) {
         int code;
@@ -409,6 +436,22 @@ int dg_init_B(dg_grid* g) {
     if (!(g->d.mhd && g->d.dim >= 2)) return DG_OK;
     launch_init_B(g->d.dim, soa(g->W, g->ce.size), g->B, g->ce, g->fe, g->s);
     HIPCHK(hipStreamSynchronize(g->s));
+    return DG_OK;
+}
+
+int dg_timing(dg_grid* g, int enable, double* step_ms, long* steps) {
+    if (!g) return DG_ERR_ARG;
+    if (step_ms) *step_ms = g->step_ms;
+    if (steps) *steps = g->timed_steps;
+    if (enable >= 0) {
+        if (enable && !g->ev0) {
+            HIPCHK(hipEventCreate(&g->ev0));
+            HIPCHK(hipEventCreate(&g->ev1));
+        }
+        g->timing = enable != 0;
+        g->step_ms = 0;
+        g->timed_steps = 0;
+    }
     return DG_OK;
 }
 

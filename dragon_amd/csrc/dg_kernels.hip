@@ -278,6 +278,225 @@ KB __global__ void k_flux(Soa HL, Soa HR, Soa F, Ext ce, Box box, double dt_d, P
     store_u<R>(F, m, f, MHD);
 }
 
+// ========================================================= fused sweeps
+// Flux along one axis with the half states rebuilt in registers: MUSCL
+// (+ normal-B override) and, for the corrector pass, the CTU correction of
+// the cell (CTU.cpp:65-129 / 162-202 for MHD, 264-278 for hydro 2D).  The
+// half-state arrays of the reference never touch HBM.  Strided axes walk a
+// chunk of faces per thread, carrying the right state of the previous cell in
+// registers; the contiguous axis hands it to the next lane with a wave64
+// shuffle (63 faces per wavefront).
+struct SweepArgs {
+    Soa W;
+    const double* B[3];
+    const double* Bh[3];
+    const double* E0[3];
+    Soa Fp[3];             // predictor fluxes (corrector pass input)
+    Soa Fo;                // output fluxes of this axis
+    Ext ce, fe;
+    int lo[3], hi[3];      // faces [lo, hi) per dim; face f of axis AX sits between cells f-1 and f
+    double dt_d[3];
+    Phys ph;
+    unsigned long long* err;
+    int stage, chunk;
+};
+
+// CTU correction of the half states of one cell (double precision).
+template <int D, bool MHD, int AX>
+__device__ __forceinline__ void ctu_correct(const SweepArgs& a, long m, long q, double dBx, double dBy, double dBz,
+                                            P<double>& L, P<double>& Rr) {
+    const Ext& ce = a.ce;
+    const Phys& ph = a.ph;
+    auto fd = [&](int ax) { return usub<MHD>(load_u(a.Fp[ax], m, MHD), load_u(a.Fp[ax], m + ce.s[ax], MHD)); };
+    if constexpr (!MHD) {  // correctState (hydro 2D): minus the transverse flux difference
+        constexpr int T = AX == 0 ? 1 : 0;
+        U<double> t = umul<false>(usub<false>(load_u(a.Fp[T], m + ce.s[T], false), load_u(a.Fp[T], m, false)), 0.5 * a.dt_d[T]);
+        L = prim<false>(usub<false>(cons<false>(L, ph), t), ph);
+        Rr = prim<false>(usub<false>(cons<false>(Rr, ph), t), ph);
+        return;
+    } else {
+        const Ext& fe = a.fe;
+        const long fx = fe.s[0], fy = fe.s[1], fz = fe.s[2];
+        const double k8 = ph.k1_8pi;
+        const P<double> Wc = load_p<double>(a.W, m, true);
+        const double dt_dx = a.dt_d[0], dt_dy = a.dt_d[1], dt_dz = a.dt_d[2];
+        const double *E0x = a.E0[0], *E0y = a.E0[1], *E0z = a.E0[2];
+        U<double> corr;
+        if (D == 2) {
+            if (AX == 0) {
+                corr = umul<true>(umul<true>(fd(1), 0.5), dt_dy);
+                corr.Bx = corr.By = corr.Bz = 0;
+                double lim_z = minmod(-dBy, dBx);
+                corr.E += k8 * (Wc.Bz * Wc.vz * lim_z);
+                corr.Bz = 0.5 * dt_dy * (E0x[q + fy] - E0x[q]) + 0.5 * Wc.vz * lim_z;
+            } else {
+                corr = umul<true>(umul<true>(fd(0), 0.5), dt_dx);
+                corr.Bx = corr.By = corr.Bz = 0;
+                double lim_z = minmod(-dBx, dBy);
+                corr.E += k8 * (Wc.Bz * Wc.vz * lim_z);
+                corr.Bz = -0.5 * dt_dx * (E0y[q + fx] - E0y[q]) + 0.5 * Wc.vz * lim_z;
+            }
+        } else if (AX == 0) {
+            corr = umul<true>(uadd<true>(umul<true>(fd(1), dt_dy), umul<true>(fd(2), dt_dz)), 0.5);
+            corr.Bx = corr.By = corr.Bz = 0;
+            double lim_y = minmod(-dBz, dBx), lim_z = minmod(-dBy, dBx);
+            corr.E += k8 * (Wc.By * Wc.vy * lim_y + Wc.Bz * Wc.vz * lim_z);
+            corr.By = -0.25 * dt_dz * (E0x[q + fz] - E0x[q] + E0x[q + fy + fz] - E0x[q + fy]) + 0.5 * Wc.vy * lim_y;
+            corr.Bz = 0.25 * dt_dy * (E0x[q + fy] - E0x[q] + E0x[q + fy + fz] - E0x[q + fz]) + 0.5 * Wc.vz * lim_z;
+        } else if (AX == 1) {
+            corr = umul<true>(uadd<true>(umul<true>(fd(0), dt_dx), umul<true>(fd(2), dt_dz)), 0.5);
+            corr.Bx = corr.By = corr.Bz = 0;
+            double lim_x = minmod(-dBz, dBy), lim_z = minmod(-dBx, dBy);
+            corr.E += k8 * (Wc.Bx * Wc.vx * lim_x + Wc.Bz * Wc.vz * lim_z);
+            corr.Bx = 0.25 * dt_dz * (E0y[q + fz] - E0y[q] + E0y[q + fx + fz] - E0y[q + fx]) + 0.5 * Wc.vx * lim_x;
+            corr.Bz = -0.25 * dt_dx * (E0y[q + fx] - E0y[q] + E0y[q + fx + fz] - E0y[q + fz]) + 0.5 * Wc.vz * lim_z;
+        } else {
+            corr = umul<true>(uadd<true>(umul<true>(fd(0), dt_dx), umul<true>(fd(1), dt_dy)), 0.5);
+            corr.Bx = corr.By = corr.Bz = 0;
+            double lim_x = minmod(-dBy, dBz), lim_y = minmod(-dBx, dBz);
+            corr.E += k8 * (Wc.Bx * Wc.vx * lim_x + Wc.By * Wc.vy * lim_y);
+            corr.Bx = -0.25 * dt_dy * (E0z[q + fy] - E0z[q] + E0z[q + fx + fy] - E0z[q + fx]) + 0.5 * Wc.vx * lim_x;
+            corr.By = 0.25 * dt_dx * (E0z[q + fx] - E0z[q] + E0z[q + fx + fy] - E0z[q + fy]) + 0.5 * Wc.vy * lim_y;
+        }
+        U<double> uL = uadd<true>(cons<true>(L, ph), corr), uR = uadd<true>(cons<true>(Rr, ph), corr);
+        const double* Bh = a.Bh[AX];
+        const long e = fe.s[AX];
+        if (AX == 0) { uL.Bx = Bh[q]; uR.Bx = Bh[q + e]; }
+        else if (AX == 1) { uL.By = Bh[q]; uR.By = Bh[q + e]; }
+        else { uL.Bz = Bh[q]; uR.Bz = Bh[q + e]; }
+        L = prim<true>(uL, ph);
+        Rr = prim<true>(uR, ph);
+    }
+}
+
+template <class R> __device__ __forceinline__ P<R> p_cast(const P<double>& w) {
+    return P<R>{R(w.rho), R(w.p), R(w.vx), R(w.vy), R(w.vz), R(w.Bx), R(w.By), R(w.Bz)};
+}
+template <class R> __device__ __forceinline__ P<double> p_wide(const P<R>& w) {
+    return P<double>{double(w.rho), double(w.p), double(w.vx), double(w.vy), double(w.vz), double(w.Bx), double(w.By), double(w.Bz)};
+}
+
+// Half states of cell (i,j,k) along AX, in the global frame.
+template <int D, bool MHD, class R, int AX, bool FINAL>
+__device__ __forceinline__ void cell_states(const SweepArgs& a, int i, int j, int k, P<R>& L, P<R>& Rr) {
+    const Ext& ce = a.ce;
+    const Ext& fe = a.fe;
+    const long m = ce.at(i, j, k);
+    const int idx[3] = {i, j, k};
+    const int g = ce.g;
+    const P<R> wc = load_p<R>(a.W, m, MHD);
+    double bx = 0, by = 0, bz = 0;
+    long q = 0;
+    if (MHD && D >= 2) {
+        q = fe.at(i, j, k);
+        bx = (a.B[0][q + fe.s[0]] - a.B[0][q]) * a.dt_d[0];
+        by = (a.B[1][q + fe.s[1]] - a.B[1][q]) * a.dt_d[1];
+        if (D == 3) bz = (a.B[2][q + fe.s[2]] - a.B[2][q]) * a.dt_d[2];
+    }
+    if (idx[AX] >= -g + 1 && idx[AX] < ce.n[AX] + g - 1) {
+        const long off = ce.s[AX];
+        const P<R> wl = load_p<R>(a.W, m - off, MHD), wr = load_p<R>(a.W, m + off, MHD);
+        if (AX == 0) {
+            muscl<MHD, R>(wl, L, wc, Rr, wr, R(a.dt_d[0]), R(bx), R(by), R(bz), a.ph);
+        } else if (AX == 1) {
+            P<R> l, r;
+            muscl<MHD, R>(swapXY(wl), l, swapXY(wc), r, swapXY(wr), R(a.dt_d[1]), R(by), R(bx), R(bz), a.ph);
+            L = swapXY(l); Rr = swapXY(r);
+        } else {
+            P<R> l, r;
+            muscl<MHD, R>(swapXZ(wl), l, swapXZ(wc), r, swapXZ(wr), R(a.dt_d[2]), R(bz), R(by), R(bx), a.ph);
+            L = swapXZ(l); Rr = swapXZ(r);
+        }
+    } else {
+        L = wc; Rr = wc;
+    }
+    if (MHD && D >= 2) {  // copyFaceFields: normal B := face B
+        const double* Bn = a.B[AX];
+        const R b0 = R(Bn[q]), b1 = R(Bn[q + fe.s[AX]]);
+        if (AX == 0) { L.Bx = b0; Rr.Bx = b1; }
+        else if (AX == 1) { L.By = b0; Rr.By = b1; }
+        else { L.Bz = b0; Rr.Bz = b1; }
+    }
+    if constexpr (FINAL) {
+        P<double> l = p_wide(L), r = p_wide(Rr);
+        ctu_correct<D, MHD, AX>(a, m, q, bx, by, bz, l, r);
+        L = p_cast<R>(l);
+        Rr = p_cast<R>(r);
+    }
+}
+
+template <bool MHD, class R, int AX>
+__device__ __forceinline__ void face_flux(const SweepArgs& a, P<R> l, P<R> r, long m) {
+    if (AX == 1) { l = swapXY(l); r = swapXY(r); }
+    if (AX == 2) { l = swapXZ(l); r = swapXZ(r); }
+    int fail = RF_OK;
+    U<R> f = riemann_x<MHD, R>(l, r, R(a.dt_d[AX]), a.ph, &fail);
+    if (fail) report(a.err, a.stage, (unsigned long long)m, fail == RF_BAD_INPUT ? E_RIEMANN_INPUT : E_RIEMANN_SOLUTION);
+    if (AX == 1) f = swapXY(f);
+    if (AX == 2) f = swapXZ(f);
+    store_u<R>(a.Fo, m, f, MHD);
+}
+
+// Strided axis: one thread walks `chunk` consecutive faces.
+template <int D, bool MHD, class R, int AX, bool FINAL>
+KB __global__ void k_sweep_march(SweepArgs a) {
+    constexpr int T0 = AX == 0 ? 1 : 0;            // slower transverse dim (3D only)
+    constexpr int T1 = D == 3 ? (AX == 2 ? 1 : 2) : (AX == 0 ? 1 : 0);  // fastest transverse dim
+    const long n1 = a.hi[T1] - a.lo[T1];
+    const long n0 = D == 3 ? a.hi[T0] - a.lo[T0] : 1;
+    const int len = a.hi[AX] - a.lo[AX];
+    const int nch = (len + a.chunk - 1) / a.chunk;
+    const long t = tid();
+    if (t >= (long)nch * n0 * n1) return;
+    int idx[3] = {0, 0, 0};
+    idx[T1] = a.lo[T1] + (int)(t % n1);
+    long r = t / n1;
+    if (D == 3) { idx[T0] = a.lo[T0] + (int)(r % n0); r /= n0; }
+    const int f0 = a.lo[AX] + (int)r * a.chunk;
+    const int f1 = min(a.hi[AX], f0 + a.chunk);
+    P<R> Lp, Rp, Lc, Rc;
+    idx[AX] = f0 - 1;
+    cell_states<D, MHD, R, AX, FINAL>(a, idx[0], idx[1], idx[2], Lp, Rp);
+    for (int f = f0; f < f1; f++) {
+        idx[AX] = f;
+        cell_states<D, MHD, R, AX, FINAL>(a, idx[0], idx[1], idx[2], Lc, Rc);
+        face_flux<MHD, R, AX>(a, Rp, Lc, a.ce.at(idx[0], idx[1], idx[2]));
+        Rp = Rc;
+    }
+}
+
+template <class R> __device__ __forceinline__ P<R> shfl_up1(const P<R>& w) {
+    return P<R>{__shfl_up(w.rho, 1), __shfl_up(w.p, 1), __shfl_up(w.vx, 1), __shfl_up(w.vy, 1),
+                __shfl_up(w.vz, 1), __shfl_up(w.Bx, 1), __shfl_up(w.By, 1), __shfl_up(w.Bz, 1)};
+}
+
+// Contiguous axis: lane l of a wavefront rebuilds cell c0-1+l; lanes 1..63
+// take the left state from the lane below.
+template <int D, bool MHD, class R, int AX, bool FINAL>
+KB __global__ void k_sweep_wave(SweepArgs a) {
+    constexpr int T0 = 0;                 // slowest transverse dim
+    constexpr int T1 = D == 3 ? 1 : 0;    // faster transverse dim
+    const int lane = threadIdx.x & 63;
+    const long wave = tid() >> 6;
+    const int len = a.hi[AX] - a.lo[AX];
+    const int nseg = (len + 62) / 63;
+    const long n1 = D == 3 ? a.hi[T1] - a.lo[T1] : 1;
+    const long n0 = a.hi[T0] - a.lo[T0];
+    if (wave >= (long)nseg * n0 * n1) return;  // uniform per wavefront
+    int idx[3] = {0, 0, 0};
+    const int seg = (int)(wave % nseg);
+    long r = wave / nseg;
+    if (D == 3) { idx[T1] = a.lo[T1] + (int)(r % n1); r /= n1; }
+    idx[T0] = a.lo[T0] + (int)r;
+    const int c = a.lo[AX] + seg * 63 - 1 + lane;
+    const bool live = c < a.hi[AX];
+    P<R> L = pzero<R>(), Rr = pzero<R>();
+    idx[AX] = c;
+    if (live) cell_states<D, MHD, R, AX, FINAL>(a, idx[0], idx[1], idx[2], L, Rr);
+    const P<R> Rl = shfl_up1(Rr);
+    if (lane >= 1 && live) face_flux<MHD, R, AX>(a, Rl, L, a.ce.at(idx[0], idx[1], idx[2]));
+}
+
 // ===================================================== CTU hydro (2D/3D)
 // correctState (CTU.cpp:264-301): states along SAX corrected by the flux
 // difference of FAX.
@@ -747,8 +966,104 @@ static Box face_box(int D, const Ext& ce, int ax, int along_lo, int along_hi_inc
     return make_box(D, lo[0], hi[0], lo[1], hi[1], lo[2], hi[2]);
 }
 
+// One sweep launch for axis AX over the face box b.
+template <int D, bool MHD, class R, bool FINAL, int AX>
+static void sweep_t(const StepCtx& c, const Box& b, const Soa& out, int stage, hipStream_t s) {
+    SweepArgs a{};
+    a.W = c.W;
+    for (int q = 0; q < 3; q++) {
+        a.B[q] = c.B[q]; a.Bh[q] = c.Bh[q]; a.E0[q] = c.E0[q]; a.Fp[q] = c.Fp[q];
+        a.lo[q] = b.lo[q]; a.hi[q] = b.hi[q];
+        a.dt_d[q] = c.dt / c.d[q];
+    }
+    a.Fo = out; a.ce = c.ce; a.fe = c.fe; a.ph = c.ph; a.err = c.err; a.stage = stage;
+    const long len = b.hi[AX] - b.lo[AX];
+    long ntrans = 1;
+    for (int d = 0; d < D; d++)
+        if (d != AX) ntrans *= b.hi[d] - b.lo[d];
+    if (len <= 0 || ntrans <= 0) return;
+    if (AX == D - 1) {
+        const long waves = ((len + 62) / 63) * ntrans;
+        a.chunk = 63;
+        hipLaunchKernelGGL((k_sweep_wave<D, MHD, R, AX, FINAL>), dim3(blocks(waves * 64)), dim3(256), 0, s, a);
+    } else {
+        // enough independent walkers to fill the chip (~2^18 threads), each
+        // walking at least 4 faces
+        const long want = 1L << 18;
+        long nch = (want + ntrans - 1) / ntrans;
+        long chunk = (len + nch - 1) / nch;
+        if (chunk < 4) chunk = 4;
+        if (chunk > len) chunk = len;
+        nch = (len + chunk - 1) / chunk;
+        a.chunk = (int)chunk;
+        hipLaunchKernelGGL((k_sweep_march<D, MHD, R, AX, FINAL>), dim3(blocks(nch * ntrans)), dim3(256), 0, s, a);
+    }
+}
+template <int D, bool MHD, class R, bool FINAL>
+static void sweep(const StepCtx& c, int ax, const Box& b, const Soa& out, int stage, hipStream_t s) {
+    if (ax == 0) sweep_t<D, MHD, R, FINAL, 0>(c, b, out, stage, s);
+    else if (ax == 1) { if constexpr (D >= 2) sweep_t<D, MHD, R, FINAL, 1>(c, b, out, stage, s); }
+    else { if constexpr (D >= 3) sweep_t<D, MHD, R, FINAL, 2>(c, b, out, stage, s); }
+}
+
+// The step with fused sweeps (MHD 2D/3D, hydro 2D): predictor fluxes ->
+// [MHD: corner EMF, half-step faces, half-step reference EMF] -> corrector
+// fluxes (CTU correction inside) -> [MHD: EMF, Faraday] -> apply.
+template <int D, bool MHD, class R>
+static void step_fused(const StepCtx& c, hipStream_t s) {
+    const Ext& ce = c.ce;
+    const double dt_d[3] = {c.dt / c.d[0], c.dt / c.d[1], c.dt / c.d[2]};
+    if (c.ctu) {
+        for (int ax = 0; ax < D; ax++) {
+            Box b = MHD ? face_box(D, ce, ax, -1, 1, -2, 2) : face_box(D, ce, ax, 0, 0, -1, 1);
+            sweep<D, MHD, R, false>(c, ax, b, c.Fp[ax], ST_PRED_X + ax, s);
+        }
+        if constexpr (MHD) {
+            ERef none{nullptr, nullptr, nullptr};
+            Box eb = make_box(D, -1, ce.n[0] + 2, -1, ce.n[1] + 2, -1, ce.n[2] + 2);
+            if constexpr (D == 2)
+                hipLaunchKernelGGL((k_emf2<true>), dim3(blocks(eb.count())), dim3(256), 0, s, c.Fp[0], c.Fp[1], c.W, none, ce, c.fe, eb, 1,
+                                   c.E0[0], c.E0[1], c.E0[2]);
+            else
+                hipLaunchKernelGGL((k_emf3<true>), dim3(blocks(eb.count())), dim3(256), 0, s, c.Fp[0], c.Fp[1], c.Fp[2], c.W, none, ce,
+                                   c.fe, eb, 1, c.E0[0], c.E0[1], c.E0[2]);
+            hipLaunchKernelGGL((k_faraday<D>), dim3(blocks(c.fe.size)), dim3(256), 0, s, c.E0[0], c.E0[1], c.E0[2], c.B[0], c.B[1],
+                               c.B[2], c.Bh[0], c.Bh[1], c.Bh[2], c.fe, 1, 0.5 * dt_d[0], 0.5 * dt_d[1], 0.5 * dt_d[2]);
+            Box cb = make_box(D, -1, ce.n[0] + 1, -1, ce.n[1] + 1, -1, ce.n[2] + 1);
+            hipLaunchKernelGGL((k_whalf<D>), dim3(blocks(cb.count())), dim3(256), 0, s, c.W, c.Fp[0], c.Fp[1], c.Fp[2], c.Bh[0],
+                               c.Bh[1], c.Bh[2], ce, c.fe, cb, 0.5 * dt_d[0], 0.5 * dt_d[1], 0.5 * dt_d[2], c.ph, c.Eh[0], c.Eh[1],
+                               c.Eh[2], c.err);
+        }
+    }
+    for (int ax = 0; ax < D; ax++) {
+        Box b = MHD ? face_box(D, ce, ax, 0, 0, -1, 1) : face_box(D, ce, ax, 0, 0, 0, 0);
+        if (c.ctu) sweep<D, MHD, R, true>(c, ax, b, c.F[ax], ST_FLUX_X + ax, s);
+        else sweep<D, MHD, R, false>(c, ax, b, c.F[ax], ST_FLUX_X + ax, s);
+    }
+    if constexpr (MHD) {
+        Box eb = make_box(D, 0, ce.n[0] + 1, 0, ce.n[1] + 1, 0, ce.n[2] + 1);
+        ERef eh{c.Eh[0], c.Eh[1], c.Eh[2]};
+        if constexpr (D == 2)
+            hipLaunchKernelGGL((k_emf2<false>), dim3(blocks(eb.count())), dim3(256), 0, s, c.F[0], c.F[1], c.W, eh, ce, c.fe, eb, c.ctu,
+                               c.E0[0], c.E0[1], c.E0[2]);
+        else
+            hipLaunchKernelGGL((k_emf3<false>), dim3(blocks(eb.count())), dim3(256), 0, s, c.F[0], c.F[1], c.F[2], c.W, eh, ce, c.fe, eb,
+                               c.ctu, c.E0[0], c.E0[1], c.E0[2]);
+        hipLaunchKernelGGL((k_faraday<D>), dim3(blocks(c.fe.size)), dim3(256), 0, s, c.E0[0], c.E0[1], c.E0[2], c.B[0], c.B[1], c.B[2],
+                           c.Bn[0], c.Bn[1], c.Bn[2], c.fe, 0, dt_d[0], dt_d[1], dt_d[2]);
+    }
+    hipLaunchKernelGGL((k_apply<D, MHD>), dim3(blocks(ce.size)), dim3(256), 0, s, c.W, c.Wn, c.F[0], c.F[1], c.F[2], c.Bn[0], c.Bn[1],
+                       c.Bn[2], ce, c.fe, dt_d[0], dt_d[1], dt_d[2], c.ph, c.err);
+}
+
 template <int D, bool MHD, class R>
 static void step_t(const StepCtx& c, hipStream_t s) {
+    if constexpr ((MHD && D >= 2) || (!MHD && D == 2)) {
+        if (c.fused) {
+            step_fused<D, MHD, R>(c, s);
+            return;
+        }
+    }
     const Ext& ce = c.ce;
     const double dt_d[3] = {c.dt / c.d[0], c.dt / c.d[1], c.dt / c.d[2]};
     const int g = ce.g;

@@ -12,6 +12,7 @@ struct StepCtx {
     int dim;
     bool mhd, mixed;
     int ctu, cfl_mode;
+    bool fused;           // fused sweeps (no half-state arrays) where available
     Ext ce, fe;
     Phys ph;
     double d[3];
@@ -21,7 +22,8 @@ struct StepCtx {
     double* Bn[3];        // next face B
     double* Bh[3];        // half-step face B (CTU MHD)
     Soa HS[6];            // xL, xR, yL, yR, zL, zR
-    Soa F[3];             // fluxes (predictor, then final)
+    Soa F[3];             // corrector (final) fluxes; reference-structured path: both
+    Soa Fp[3];            // predictor fluxes of the fused path
     Soa T[5];             // 3D hydro corner fluxes
     double* E0[3];        // corner EMF (predictor, then final)
     double* Eh[3];        // half-step reference EMF

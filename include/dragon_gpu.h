@@ -128,6 +128,10 @@ int dg_init_B(dg_grid* g);
 const char* dg_last_error(const dg_grid* g);
 /* HIP stream the handle's kernels run on (as hipStream_t). */
 void* dg_stream(dg_grid* g);
+/* Live timing of the step kernels with HIP events on the handle's stream.
+ * enable: 1 start (resets), 0 stop, -1 read only.  step_ms: summed device
+ * time from the first to the last kernel of each timed step. */
+int dg_timing(dg_grid* g, int enable, double* step_ms, long* steps);
 /* Diagnostics: max over interior cells of |div B| * min(dx) / max|B_face| (0 for hydro / 1D). */
 int dg_divb(dg_grid* g, double* out);
 
