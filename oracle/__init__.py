@@ -102,6 +102,11 @@ def _load():
         "oc_upwind_corr": (C.c_double, [C.c_double] * 5),
         "oc_protect_thermal": (C.c_int, [cp, _dp]),
         "oc_face_mask": (C.c_int, [C.c_char_p]),
+        "oc_ct_electric": (None, [cp] + [C.c_int] * 5 + [_dp, _dp, _dp, _dp, C.c_int, _dp]),
+        "oc_ct_body_electric": (None, [cp] + [C.c_int] * 5 + [_dp, C.c_int, _dp]),
+        "oc_ct_faraday": (None, [cp] + [C.c_int] * 5 + [_dp, C.c_double, C.c_double, C.c_double, C.c_int, _dp]),
+        "oc_ct_body_fields": (None, [cp] + [C.c_int] * 5 + [_dp, _dp]),
+        "oc_cfl_speed": (C.c_double, [cp, _dp, C.c_double, C.c_double, C.c_double]),
     }
     for n, (r, a) in sig.items():
         f = getattr(lib, n)
@@ -317,6 +322,44 @@ def protect_thermal(cfg: Cfg, w) -> bool:
 
 def face_mask(s: str) -> int:
     return lib().oc_face_mask(s.encode())
+
+
+def cfl_speed(cfg: Cfg, w, dx, dy=0.0, dz=0.0):
+    c = cfg.c()
+    return lib().oc_cfl_speed(C.byref(c), _ptr(np.asarray(w, float)), dx, dy, dz)
+
+
+def _shape5(shp):
+    s = list(shp) + [1] * (3 - len(shp))
+    return len(shp), s[0], s[1], s[2]
+
+
+def ct_electric(cfg: Cfg, shp, g, FX, FY, FZ, ref, gg, E):
+    """computeElectric (+ upwindElectric if ref is not None) into E (in place)."""
+    c = cfg.c()
+    d, nx, ny, nz = _shape5(shp)
+    FZ = FY if FZ is None else FZ
+    lib().oc_ct_electric(C.byref(c), d, nx, ny, nz, g, _ptr(FX), _ptr(FY), _ptr(FZ), None if ref is None else _ptr(ref), gg,
+                         _ptr(E))
+
+
+def ct_body_electric(cfg: Cfg, shp, g, W, gg, E):
+    c = cfg.c()
+    d, nx, ny, nz = _shape5(shp)
+    lib().oc_ct_body_electric(C.byref(c), d, nx, ny, nz, g, _ptr(W), gg, _ptr(E))
+
+
+def ct_faraday(cfg: Cfg, shp, g, E, dts, gg, B):
+    c = cfg.c()
+    d, nx, ny, nz = _shape5(shp)
+    dts = list(dts) + [0.0] * (3 - len(dts))
+    lib().oc_ct_faraday(C.byref(c), d, nx, ny, nz, g, _ptr(E), dts[0], dts[1], dts[2], gg, _ptr(B))
+
+
+def ct_body_fields(cfg: Cfg, shp, g, B, W):
+    c = cfg.c()
+    d, nx, ny, nz = _shape5(shp)
+    lib().oc_ct_body_fields(C.byref(c), d, nx, ny, nz, g, _ptr(B), _ptr(W))
 
 
 # ------------------------------------------------------------ reference

@@ -187,5 +187,27 @@ int oc_is_physical(const oc_cfg* c, const double* w8) { CFG = to_cfg(c); return 
 double oc_upwind_corr(double m, double fm, double fp, double rm, double rp) { return upwind_corr(m, fm, fp, rm, rp); }
 int oc_protect_thermal(const oc_cfg* c, const double* w8) { CFG = to_cfg(c); return should_protect_thermal(P8(w8)) ? 1 : 0; }
 int oc_face_mask(const char* s) { return face_mask(s); }
+void oc_ct_electric(const oc_cfg* c, int dim, int nx, int ny, int nz, int g, const double* FX, const double* FY, const double* FZ,
+                    const double* ref, int gg, double* E) {
+    CFG = to_cfg(c);
+    test_electric(dim, nx, ny, nz, g, FX, FY, FZ, ref, gg, E);
+}
+void oc_ct_body_electric(const oc_cfg* c, int dim, int nx, int ny, int nz, int g, const double* W, int gg, double* E) {
+    CFG = to_cfg(c);
+    test_body_electric(dim, nx, ny, nz, g, W, gg, E);
+}
+void oc_ct_faraday(const oc_cfg* c, int dim, int nx, int ny, int nz, int g, const double* E, double dx, double dy, double dz, int gg,
+                   double* B) {
+    CFG = to_cfg(c);
+    test_faraday(dim, nx, ny, nz, g, E, dx, dy, dz, gg, B);
+}
+void oc_ct_body_fields(const oc_cfg* c, int dim, int nx, int ny, int nz, int g, const double* B, double* W) {
+    CFG = to_cfg(c);
+    test_body_fields(dim, nx, ny, nz, g, B, W);
+}
+double oc_cfl_speed(const oc_cfg* c, const double* w8, double dx, double dy, double dz) {
+    CFG = to_cfg(c);
+    return test_cfl_speed(P8(w8), dx, dy, dz);
+}
 
 }  // extern "C"

@@ -9,6 +9,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
+#include <cstring>
 #include <exception>
 #include <thread>
 
@@ -1068,4 +1069,72 @@ long Grid::advance(double dt, bool check_cfl, long* failures, long max_steps) {
     return steps;
 }
 
+}  // namespace oracle
+
+// ---------------------------------------------------- test entry points
+// CT pieces on caller arrays in ExtendedArray layout (8 doubles per cell /
+// flux record, 3 per face/edge), for the function-level checks.
+namespace oracle {
+template <class T>
+static void load_arr(Arr<T>& A, const double* p) { std::memcpy(A.a.data(), p, A.a.size() * sizeof(T)); }
+template <class T>
+static void save_arr(const Arr<T>& A, double* p) { std::memcpy(p, A.a.data(), A.a.size() * sizeof(T)); }
+static_assert(sizeof(Prim) == 64 && sizeof(Cons) == 64 && sizeof(V3) == 24, "record layout");
+
+void test_electric(int dim, int nx, int ny, int nz, int g, const double* FX, const double* FY, const double* FZ,
+                   const double* ref, int gg, double* E) {
+    if (dim == 2) {
+        Arr<Cons> fx(nx, ny, g), fy(nx, ny, g);
+        load_arr(fx, FX); load_arr(fy, FY);
+        Arr<V3> e(nx + 1, ny + 1, g);
+        load_arr(e, E);
+        compute_electric2D(e, fx, fy, gg);
+        if (ref) { Arr<V3> r(nx + 1, ny + 1, g); load_arr(r, ref); upwind_electric2D(e, fx, fy, r, gg); }
+        save_arr(e, E);
+    } else {
+        Arr<Cons> fx(nx, ny, nz, g), fy(nx, ny, nz, g), fz(nx, ny, nz, g);
+        load_arr(fx, FX); load_arr(fy, FY); load_arr(fz, FZ);
+        Arr<V3> e(nx + 1, ny + 1, nz + 1, g);
+        load_arr(e, E);
+        compute_electric3D(e, fx, fy, fz, gg);
+        if (ref) { Arr<V3> r(nx + 1, ny + 1, nz + 1, g); load_arr(r, ref); upwind_electric3D(e, fx, fy, fz, r, gg); }
+        save_arr(e, E);
+    }
+}
+void test_body_electric(int dim, int nx, int ny, int nz, int g, const double* W, int gg, double* E) {
+    if (dim == 2) {
+        Arr<Prim> w(nx, ny, g); load_arr(w, W);
+        Arr<V3> e(nx + 1, ny + 1, g); load_arr(e, E);
+        body_electric2D(w, e, gg); save_arr(e, E);
+    } else {
+        Arr<Prim> w(nx, ny, nz, g); load_arr(w, W);
+        Arr<V3> e(nx + 1, ny + 1, nz + 1, g); load_arr(e, E);
+        body_electric3D(w, e, gg); save_arr(e, E);
+    }
+}
+void test_faraday(int dim, int nx, int ny, int nz, int g, const double* E, double dx, double dy, double dz, int gg, double* B) {
+    if (dim == 2) {
+        Arr<V3> e(nx + 1, ny + 1, g), b(nx + 1, ny + 1, g);
+        load_arr(e, E); load_arr(b, B);
+        faraday2D(e, b, dx, dy, gg); save_arr(b, B);
+    } else {
+        Arr<V3> e(nx + 1, ny + 1, nz + 1, g), b(nx + 1, ny + 1, nz + 1, g);
+        load_arr(e, E); load_arr(b, B);
+        faraday3D(e, b, dx, dy, dz, gg); save_arr(b, B);
+    }
+}
+void test_body_fields(int dim, int nx, int ny, int nz, int g, const double* B, double* W) {
+    if (dim == 2) {
+        Arr<V3> b(nx + 1, ny + 1, g); load_arr(b, B);
+        Arr<Prim> w(nx, ny, g); load_arr(w, W);
+        compute_body_fields2D(b, w); save_arr(w, W);
+    } else {
+        Arr<V3> b(nx + 1, ny + 1, nz + 1, g); load_arr(b, B);
+        Arr<Prim> w(nx, ny, nz, g); load_arr(w, W);
+        compute_body_fields3D(b, w); save_arr(w, W);
+    }
+}
+}  // namespace oracle
+namespace oracle {
+double test_cfl_speed(const Prim& W, double dx, double dy, double dz) { return cfl_speed(W, dx, dy, dz); }
 }  // namespace oracle

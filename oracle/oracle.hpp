@@ -205,3 +205,15 @@ bool should_protect_thermal(const Prim& w);
 double upwind_corr(double m, double fm, double fp, double rm, double rp);
 
 }  // namespace oracle
+
+namespace oracle {
+// Function-level entry points for the checks against the reference's own code.
+void test_electric(int dim, int nx, int ny, int nz, int g, const double* FX, const double* FY, const double* FZ,
+                   const double* ref, int gg, double* E);
+void test_body_electric(int dim, int nx, int ny, int nz, int g, const double* W, int gg, double* E);
+void test_faraday(int dim, int nx, int ny, int nz, int g, const double* E, double dx, double dy, double dz, int gg, double* B);
+void test_body_fields(int dim, int nx, int ny, int nz, int g, const double* B, double* W);
+}  // namespace oracle
+namespace oracle {
+double test_cfl_speed(const Prim& W, double dx, double dy, double dz);
+}  // namespace oracle
