@@ -19,6 +19,8 @@ import os
 import sys
 import time
 
+import numpy as np
+
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
@@ -65,7 +67,7 @@ def make_state_gpu(n, seed, torch, dev, mach=1.0, gamma=5.0 / 3.0, B0=0.2):
     return W, Bf
 
 
-def run_form(n, precision, W, Bf, steps, warmup, torch):
+def run_form(n, precision, W, Bf, steps, warmup, torch, rho_out=None):
     import dragon_amd as D
     d = 1.0 / n
     cfg = D.Config(mhd=1, limiter=D.LIMITER_VANLEER, ctu=1, ct_energy=D.CT_CONSV_BETA_GATED, precision=precision)
@@ -81,17 +83,48 @@ def run_form(n, precision, W, Bf, steps, warmup, torch):
     done, fails = G.advance(1e9, True, steps)
     torch.cuda.synchronize()
     t1 = time.perf_counter()
-    kms, ksteps = G.timing(0)
+    kms, ksteps = G.timing(-1)
+    kern = G.kernel_times()
+    G.timing(0)
     divb = G.divb()
+    per = {k: {"ms_per_launch": ms / max(nl, 1), "launches_per_step": nl / max(ksteps, 1)} for k, (ms, nl) in kern.items()}
     out = {
         "steps": done, "failures": fails, "wall_s": t1 - t0,
         "zone_updates_per_s": n ** 3 * done / (t1 - t0),
         "ms_per_step": 1e3 * (t1 - t0) / max(done, 1),
         "kernel_ms_per_step": kms / max(ksteps, 1),
+        "kernels": per,
         "max_divB_normalised": divb,
     }
+    if rho_out is not None:
+        Wf = torch.empty_like(W)
+        Bo = torch.empty_like(Bf)
+        G.download_device(Wf.data_ptr(), Bo.data_ptr())
+        g = 3
+        rho_out.append(Wf[0, g:g + n, g:g + n, g:g + n].clone())
+        del Wf, Bo
     del G
     return out
+
+
+def copy_probe(torch, dev, nbytes=1 << 30, reps=3):
+    """HBM copy ceiling: an elementwise read+write of nbytes each way (torch's
+    negation kernel, used nowhere else in this program), timed with HIP
+    events.  Also the known-size kernel that scales the FETCH_SIZE /
+    WRITE_SIZE counters."""
+    a = torch.ones(nbytes // 8, dtype=torch.float64, device=dev)
+    b = torch.empty_like(a)
+    torch.neg(a, out=b)
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    e0.record()
+    for _ in range(reps):
+        torch.neg(a, out=b)
+    e1.record()
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / reps
+    del a, b
+    return {"bytes_read": nbytes, "bytes_written": nbytes, "ms": ms, "GBs": 2 * nbytes / (ms * 1e-3) / 1e9}
 
 
 def cpu_baseline(n, target_s=12.0, max_steps=40):
@@ -124,6 +157,80 @@ def cpu_baseline(n, target_s=12.0, max_steps=40):
             "sample": f"C4 switch set at {n}^3, {steps} steps (BC+CFL+step), oracle C++ port, {threads} threads"}
 
 
+def bench_batch(torch, npatch=4096, steps=10, seed=20260705, baseline_patches=256):
+    """The batched-rotor workload: npatch independent 2D MHD patches (HLLD +
+    CT + CTU, gamma=1.4, minmod, outflow), one persistent launch per step
+    with per-patch dt, mixed FP32; also the FP64 batch, the launches replayed
+    from a HIP graph, and per-patch launches of the single-grid path on the
+    first `baseline_patches` patches."""
+    import dragon_amd as D
+    from dragon_amd import problems as P
+    sizes, d, cfg, make = P.rotor_batch(npatch, seed)
+    t0 = time.perf_counter()
+    ics = [make(p) for p in range(npatch)]
+    log(f"batch: {npatch} patch initial states in {time.perf_counter() - t0:.1f} s")
+    out = {"workload": f"{npatch} 2D MHD rotor patches, n log-uniform in [32,256), HLLD+CT+CTU, gamma=1.4, minmod, outflow",
+           "zones_per_step": int((sizes[:, 0] * sizes[:, 1]).sum()), "steps": steps}
+    rho = {}
+    for form, prec in (("mixed", D.MIXED), ("fp64", D.FP64)):
+        conf = D.Config(precision=prec)
+        for k, v in cfg.items():
+            setattr(conf, k, v)
+        b = D.Batch(sizes, d, conf)
+        for p, (w, B) in enumerate(ics):
+            b.set_state(p, w, B)
+        b.initialize_B_fields()
+        tiles, cls_ms = b.autotune(reps=2)
+        b.run(2)
+        b.sync()
+        ms_eager = b.run(steps, graph=False, timed=True)
+        b.sync()
+        b.run(steps, graph=True)  # capture + first replay
+        b.sync()
+        ms_graph = b.run(steps, graph=True, timed=True)
+        b.sync()
+        z = b.zones
+        out[form] = {"tiles": tiles, "autotune_class_ms": cls_ms,
+                     "persistent_zone_updates_per_s": z * steps / (ms_eager * 1e-3),
+                     "graph_zone_updates_per_s": z * steps / (ms_graph * 1e-3),
+                     "ms_per_step_persistent": ms_eager / steps, "ms_per_step_graph": ms_graph / steps}
+        rho[form] = [b.get_state(p)[0][3:-3, 3:-3, 0] for p in range(0, npatch, 97)]
+        log(f"  batch {form}: {out[form]}")
+        del b
+    num = sum(float(np.abs(a - c).sum()) for a, c in zip(rho["mixed"], rho["fp64"]))
+    den = sum(float(np.abs(c).sum()) for c in rho["fp64"])
+    out["mixed_rel_L1_rho_vs_fp64_sampled_patches"] = num / den
+    # per-patch launches of the single-grid path (host-driven, one grid per patch)
+    conf = D.Config(precision=D.MIXED)
+    for k, v in cfg.items():
+        setattr(conf, k, v)
+    grids = []
+    zsub = 0
+    for p in range(baseline_patches):
+        n = int(sizes[p, 0])
+        G = D.Grid2D(n, n, float(d[p, 0]), float(d[p, 1]), 1, conf)
+        G.set_state(*ics[p])
+        G.initialize_B_fields()
+        grids.append(G)
+        zsub += n * n
+    for G in grids:
+        G.advance(1e9, True, 1)
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    for G in grids:
+        G.advance(1e9, True, 2)
+    torch.cuda.synchronize()
+    dtw = time.perf_counter() - t1
+    out["per_patch_launches"] = {"patches": baseline_patches, "zone_updates_per_s": zsub * 2 / dtw,
+                                 "note": "single-grid path, one grid per patch, BC+CFL+step launched per patch with host syncs"}
+    best = max(out["mixed"]["persistent_zone_updates_per_s"], out["mixed"]["graph_zone_updates_per_s"])
+    out["speedup_vs_per_patch_launches"] = best / out["per_patch_launches"]["zone_updates_per_s"]
+    out["roofline"] = {"bound": "hbm", "achieved": round(best * B_ALG_MHD / 1e9, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                       "frac": round(best * B_ALG_MHD / 1e9 / HBM_PEAK_GBS, 5)}
+    del grids
+    return out
+
+
 def read_traffic():
     p = os.path.join(ROOT, "profiles", "traffic_c4_512.json")
     if os.path.exists(p):
@@ -142,6 +249,8 @@ def main():
     ap.add_argument("--forms", default="fp64,mixed")
     ap.add_argument("--cpu-n", type=int, default=48)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-batch", action="store_true", help="skip the batched-rotor measurement")
+    ap.add_argument("--batch-patches", type=int, default=4096)
     a = ap.parse_args()
     if a.gpus != 1:
         print(f"bench.py runs on exactly one GPU (requested --gpus {a.gpus}); the multi-GPU "
@@ -152,43 +261,63 @@ def main():
     dev = torch.device("cuda:0")
     log(f"building {a.n}^3 initial state on the GPU")
     W, Bf = make_state_gpu(a.n, a.seed, torch, dev)
+    copy = copy_probe(torch, dev)
+    log(f"copy probe: {copy}")
     forms = {}
+    rhos = {}
     for f in a.forms.split(","):
         prec = D.MIXED if f == "mixed" else D.FP64
         log(f"running form {f}: warmup {a.warmup}, timed {a.steps}")
+        keep = []
         try:
-            forms[f] = run_form(a.n, prec, W, Bf, a.steps, a.warmup, torch)
+            forms[f] = run_form(a.n, prec, W, Bf, a.steps, a.warmup, torch, keep)
+            rhos[f] = keep[0]
         except D.DragonError as e:
             forms[f] = {"error": str(e), "zone_updates_per_s": 0.0}
         log(f"  {f}: {forms[f]}")
     del W, Bf
+    if "fp64" in rhos and "mixed" in rhos:
+        # same initial state and step count; FP64 GPU is bit-identical to the oracle
+        r64 = rhos["fp64"]
+        forms["mixed"]["rel_L1_rho_vs_fp64"] = float((rhos["mixed"] - r64).abs().sum() / r64.abs().sum())
+    rhos.clear()
     torch.cuda.empty_cache()
     best = max(forms, key=lambda f: forms[f]["zone_updates_per_s"])
     r = forms[best]
     zones = a.n ** 3
     achieved = B_ALG_MHD * zones / (r["kernel_ms_per_step"] * 1e-3) / 1e9
     traffic = read_traffic()
+    dom = max(r["kernels"].items(), key=lambda kv: kv[1]["ms_per_launch"] * kv[1]["launches_per_step"]) if r.get("kernels") else None
     roof = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 5),
             "traffic": traffic.get(best, {}).get("bytes_per_step") if traffic else None,
-            "kernel": "unsplit step (all step kernels, HIP events)",
-            "alg_bytes_per_launch": B_ALG_MHD * zones}
+            "kernel": "unsplit step: its kernel sequence on one stream, HIP events around each step",
+            "alg_bytes_per_launch": B_ALG_MHD * zones,
+            "ms_per_launch": r["kernel_ms_per_step"],
+            "dominant_kernel": {"name": dom[0], **dom[1]} if dom else None,
+            "copy_ceiling_GBs": round(copy["GBs"], 1)}
     cpu = None if a.no_cpu else cpu_baseline(a.cpu_n)
     line = {
         "metric": "zone-updates/s, 3D MHD HLLD+CT 512^3, mixed-FP32 vs FP64 GPU; % HBM roofline",
         "value": r["zone_updates_per_s"], "unit": "zone-updates/s", "n_gpus": 1,
         "steps": r["steps"], "warmup": a.warmup, "ms_per_step": r["ms_per_step"],
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
-        "dtype": "f32 reconstruction/Riemann + f64 state" if best == "mixed" else "f64",
+        "dtype": "f32+f64" if best == "mixed" else "f64",
         "data": "synthetic",
         "config": {"workload": f"3D MHD turbulence {a.n}^3 periodic, CTU+HLLD+CT, van Leer, gamma=5/3, rms Mach 1, B0=0.2",
                    "grid": [a.n] * 3, "form": best},
         "roofline": roof,
         "cpu_baseline": cpu,
         "forms": forms,
+        "copy_probe": copy,
     }
     if "mixed" in forms and "fp64" in forms and forms["fp64"]["zone_updates_per_s"] > 0:
         line["mixed_speedup_vs_fp64"] = forms["mixed"]["zone_updates_per_s"] / forms["fp64"]["zone_updates_per_s"]
+    if not a.no_batch:
+        try:
+            line["batch"] = bench_batch(torch, npatch=a.batch_patches)
+        except Exception as e:  # the headline line must still print
+            line["batch"] = {"error": repr(e)}
     print(json.dumps(line), flush=True)
 
 

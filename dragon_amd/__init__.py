@@ -105,6 +105,23 @@ def lib():
             "dg_stream": (vp, [vp]),
             "dg_divb": (C.c_int, [vp, _dp]),
             "dg_timing": (C.c_int, [vp, C.c_int, _dp, C.POINTER(C.c_long)]),
+            "dg_kernel_time": (C.c_int, [vp, C.c_int, C.POINTER(C.c_char_p), _dp, C.POINTER(C.c_long)]),
+            "dg_batch_create": (C.c_int, [C.POINTER(_Desc), C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int), _dp, _dp, C.c_int,
+                                          C.POINTER(vp)]),
+            "dg_batch_destroy": (C.c_int, [vp]),
+            "dg_batch_zones": (C.c_long, [vp]),
+            "dg_batch_ghosts": (C.c_int, [vp]),
+            "dg_batch_upload": (C.c_int, [vp, C.c_int, _dp, C.c_int, _dp]),
+            "dg_batch_download": (C.c_int, [vp, C.c_int, _dp, C.c_int, _dp]),
+            "dg_batch_init_B": (C.c_int, [vp]),
+            "dg_batch_run": (C.c_int, [vp, C.c_int, C.c_int, _dp]),
+            "dg_batch_sync": (C.c_int, [vp, C.POINTER(_Status)]),
+            "dg_batch_patch_stats": (C.c_int, [vp, C.c_int, _dp, C.POINTER(C.c_long), C.POINTER(C.c_long), C.POINTER(C.c_int)]),
+            "dg_batch_set_tiles": (C.c_int, [vp, C.POINTER(C.c_int), C.c_int]),
+            "dg_batch_get_tiles": (C.c_int, [vp, C.POINTER(C.c_int), C.c_int]),
+            "dg_batch_autotune": (C.c_int, [vp, C.c_int, C.POINTER(C.c_int), _dp]),
+            "dg_batch_last_error": (C.c_char_p, [vp]),
+            "dg_batch_stream": (vp, [vp]),
         }
         for n, (r, a) in sig.items():
             f = getattr(L, n)
@@ -405,6 +422,22 @@ class Grid:
         self._chk(lib().dg_timing(self._h, int(enable), ms.ctypes.data_as(_dp), C.byref(n)))
         return float(ms[0]), int(n.value)
 
+    def kernel_times(self) -> dict:
+        """Per-kernel-class device time recorded while timing is on:
+        {name: (total ms, launches)}; classes never launched are omitted."""
+        out = {}
+        k = 0
+        while True:
+            name = C.c_char_p()
+            ms = np.zeros(1)
+            n = C.c_long(0)
+            if lib().dg_kernel_time(self._h, k, C.byref(name), ms.ctypes.data_as(_dp), C.byref(n)) != DG_OK:
+                break
+            if n.value:
+                out[name.value.decode()] = (float(ms[0]), int(n.value))
+            k += 1
+        return out
+
     def divb(self) -> float:
         out = np.zeros(1)
         self._chk(lib().dg_divb(self._h, out.ctypes.data_as(_dp)))
@@ -424,3 +457,114 @@ class Grid2D(Grid):
 class Grid3D(Grid):
     def __init__(self, nx, ny, nz, dx, dy, dz, ghosts=2, config: Config | None = None):
         super().__init__((nx, ny, nz), (dx, dy, dz), ghosts, config or Config())
+
+
+# --------------------------------------------------------------- batches
+BATCH_CLASSES = 4
+
+
+class Batch:
+    """Many independent 2D MHD patches (CTU + CT), each its own Grid2D with
+    3 ghosts and outflow (or periodic) faces, advanced together: one
+    persistent launch per step, each patch with its own CFL dt and its own
+    dt halving.  It replaces the per-child dispatch of DistGrid<T>::step
+    (Refinement/DistGrid.cpp:291-303) for patches that share no dt."""
+
+    def __init__(self, sizes, d, config: Config | None = None, bc: str = "outflow"):
+        cfg = config or Config()
+        sizes = np.asarray(sizes, dtype=np.int64).reshape(-1, 2)
+        d = np.asarray(d, dtype=np.float64).reshape(-1, 2)
+        assert len(sizes) == len(d)
+        self.n = len(sizes)
+        self.sizes = [tuple(int(v) for v in r) for r in sizes]
+        self.config = cfg
+        desc = _Desc()
+        for f in fields(Config):
+            setattr(desc, f.name, getattr(cfg, f.name))
+        desc.dim = 2
+        nx = (C.c_int * self.n)(*[r[0] for r in self.sizes])
+        ny = (C.c_int * self.n)(*[r[1] for r in self.sizes])
+        dx = np.ascontiguousarray(d[:, 0])
+        dy = np.ascontiguousarray(d[:, 1])
+        kind = {"outflow": Outflow.kind, "periodic": Periodic.kind}[bc]
+        h = C.c_void_p()
+        rc = lib().dg_batch_create(C.byref(desc), self.n, nx, ny, dx.ctypes.data_as(_dp), dy.ctypes.data_as(_dp), kind, C.byref(h))
+        if rc:
+            raise DragonError(rc, f"dg_batch_create failed ({rc})")
+        self._h = h
+        self.g = lib().dg_batch_ghosts(h)
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h:
+            lib().dg_batch_destroy(h)
+            self._h = None
+
+    def _chk(self, rc, st: _Status | None = None):
+        if rc:
+            msg = st.msg.decode() if st is not None and st.msg else (lib().dg_batch_last_error(self._h) or b"").decode()
+            raise DragonError(rc, msg, (st.i,) if st is not None else None)
+
+    @property
+    def zones(self) -> int:
+        return int(lib().dg_batch_zones(self._h))
+
+    @property
+    def stream(self):
+        return lib().dg_batch_stream(self._h)
+
+    def wshape(self, p):
+        return tuple(s + 2 * self.g for s in self.sizes[p]) + (8,)
+
+    def bshape(self, p):
+        return tuple(s + 1 + 2 * self.g for s in self.sizes[p]) + (3,)
+
+    def set_state(self, p: int, w: np.ndarray, B: np.ndarray):
+        w = np.ascontiguousarray(w, dtype=np.float64)
+        B = np.ascontiguousarray(B, dtype=np.float64)
+        assert w.shape == self.wshape(p) and B.shape == self.bshape(p)
+        self._chk(lib().dg_batch_upload(self._h, p, w.ctypes.data_as(_dp), 8, B.ctypes.data_as(_dp)))
+
+    def get_state(self, p: int):
+        w = np.zeros(self.wshape(p))
+        B = np.zeros(self.bshape(p))
+        self._chk(lib().dg_batch_download(self._h, p, w.ctypes.data_as(_dp), 8, B.ctypes.data_as(_dp)))
+        return w, B
+
+    def initialize_B_fields(self):
+        self._chk(lib().dg_batch_init_B(self._h))
+
+    def run(self, nsteps: int = 1, graph: bool = False, timed: bool = False):
+        """nsteps steps of every patch (asynchronous unless timed); returns
+        the device ms of the nsteps when timed."""
+        ms = np.zeros(1)
+        self._chk(lib().dg_batch_run(self._h, int(nsteps), int(graph), ms.ctypes.data_as(_dp) if timed else None))
+        return float(ms[0]) if timed else None
+
+    def sync(self):
+        """Wait; raise DragonError for the first patch whose step failed."""
+        st = _Status()
+        self._chk(lib().dg_batch_sync(self._h, C.byref(st)), st)
+
+    def stats(self, p: int) -> dict:
+        dt = np.zeros(1)
+        steps, fails, status = C.c_long(0), C.c_long(0), C.c_int(0)
+        self._chk(lib().dg_batch_patch_stats(self._h, p, dt.ctypes.data_as(_dp), C.byref(steps), C.byref(fails), C.byref(status)))
+        return {"dt": float(dt[0]), "steps": steps.value, "fails": fails.value, "status": status.value}
+
+    def tiles(self):
+        out = (C.c_int * BATCH_CLASSES)()
+        self._chk(lib().dg_batch_get_tiles(self._h, out, BATCH_CLASSES))
+        return list(out)
+
+    def set_tiles(self, chunks):
+        arr = (C.c_int * len(chunks))(*chunks)
+        self._chk(lib().dg_batch_set_tiles(self._h, arr, len(chunks)))
+
+    def autotune(self, reps: int = 3):
+        """Pick the x-sweep tile per size class by timing; returns (tiles, ms per class launch)."""
+        out = (C.c_int * BATCH_CLASSES)()
+        ms = np.zeros(BATCH_CLASSES)
+        self._chk(lib().dg_batch_autotune(self._h, int(reps), out, ms.ctypes.data_as(_dp)))
+        return list(out), ms.tolist()
+

@@ -19,6 +19,7 @@
 #include "dg_kernels.hpp"
 
 using namespace dg;
+using namespace dg::p64;  // the FP64 kernels; the mixed form dispatches to dg::p32 explicitly
 
 struct dg_grid {
     dg_desc d;
@@ -40,6 +41,13 @@ struct dg_grid {
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     double step_ms = 0;
     long timed_steps = 0;
+    // per-kernel timing: an event after each launch, tagged with its kernel
+    // class (-1 opens a region), resolved after the next stream sync
+    std::vector<hipEvent_t> evs;
+    std::vector<int> ev_id;
+    int n_ev = 0;
+    double kms[K_COUNT] = {};
+    long klaunch[K_COUNT] = {};
 };
 
 static const unsigned long long NO_ERR = ~0ull;
@@ -52,6 +60,30 @@ static const unsigned long long NO_ERR = ~0ull;
             return DG_ERR_HIP;                                                 \
         }                                                                      \
     } while (0)
+
+static void tm_record(dg_grid* g, int id) {
+    if (!g->timing) return;
+    if (g->n_ev == (int)g->evs.size()) {
+        hipEvent_t e;
+        if (hipEventCreate(&e) != hipSuccess) return;
+        g->evs.push_back(e);
+        g->ev_id.push_back(-1);
+    }
+    if (hipEventRecord(g->evs[g->n_ev], g->s) != hipSuccess) return;
+    g->ev_id[g->n_ev++] = id;
+}
+static void tm_mark(void* user, int id) { tm_record((dg_grid*)user, id); }
+// Call only after the stream has been synchronised.
+static void tm_flush(dg_grid* g) {
+    for (int i = 1; i < g->n_ev; i++) {
+        const int id = g->ev_id[i];
+        float ms = 0;
+        if (id < 0 || hipEventElapsedTime(&ms, g->evs[i - 1], g->evs[i]) != hipSuccess) continue;
+        g->kms[id] += ms;
+        g->klaunch[id]++;
+    }
+    g->n_ev = 0;
+}
 
 static int fail(dg_grid* g, int code, const std::string& m) {
     if (g) g->err = m;
@@ -107,6 +139,8 @@ static StepCtx make_ctx(dg_grid* g, double dt) {
     if (g->T)
         for (int q = 0; q < 5; q++) c.T[q] = soa(g->T + (long)q * g->nf * nc, nc);
     c.err = g->dev_words;
+    c.mark = g->timing ? tm_mark : nullptr;
+    c.mark_user = g;
     return c;
 }
 
@@ -182,6 +216,8 @@ int dg_destroy(dg_grid* g) {
     if (g->host_words && hipHostFree(g->host_words) != hipSuccess) rc = DG_ERR_HIP;
     if (g->ev0 && hipEventDestroy(g->ev0) != hipSuccess) rc = DG_ERR_HIP;
     if (g->ev1 && hipEventDestroy(g->ev1) != hipSuccess) rc = DG_ERR_HIP;
+    for (hipEvent_t e : g->evs)
+        if (hipEventDestroy(e) != hipSuccess) rc = DG_ERR_HIP;
     if (g->s && hipStreamDestroy(g->s) != hipSuccess) rc = DG_ERR_HIP;
     delete g;
     return rc;
@@ -308,8 +344,10 @@ int dg_apply_boundary(dg_grid* g) {
     for (auto& b : g->bcs) uncovered &= ~b.faces;
     dg_bc imp{};
     imp.kind = DG_BC_OUTFLOW; imp.faces = uncovered; imp.corners = 1; imp.gated = 0;
+    tm_record(g, -1);
     apply_one(g, imp);
     for (auto& b : g->bcs) apply_one(g, b);
+    tm_record(g, K_BC);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return fail(g, DG_ERR_HIP, hipGetErrorString(e));
     return DG_OK;
@@ -319,9 +357,12 @@ int dg_cfl_dt(dg_grid* g, double* dt, dg_status* st) {
     if (!g || !dt) return DG_ERR_ARG;
     launch_fill_u64(g->dev_words + 1, 0ull, 1, g->s);
     StepCtx c = make_ctx(g, 0.0);
+    tm_record(g, -1);
     launch_cfl(c, g->dev_words + 1, g->s);
+    tm_record(g, K_CFL);
     HIPCHK(hipMemcpyAsync(g->host_words + 1, g->dev_words + 1, sizeof(unsigned long long), hipMemcpyDeviceToHost, g->s));
     HIPCHK(hipStreamSynchronize(g->s));
+    tm_flush(g);
     unsigned long long bits = g->host_words[1];
     double m;
     memcpy(&m, &bits, sizeof m);
@@ -368,7 +409,9 @@ int dg_step(dg_grid* g, double dt, dg_status* st) {
     launch_fill_u64(g->dev_words, NO_ERR, 1, g->s);
     StepCtx c = make_ctx(g, dt);
     if (g->timing) HIPCHK(hipEventRecord(g->ev0, g->s));
-    launch_step(c, g->s);
+    tm_record(g, -1);
+    if (c.mixed) p32::launch_step(c, g->s);
+    else p64::launch_step(c, g->s);
     HIPCHK(hipGetLastError());
     if (g->timing) HIPCHK(hipEventRecord(g->ev1, g->s));
     HIPCHK(hipMemcpyAsync(g->host_words, g->dev_words, sizeof(unsigned long long), hipMemcpyDeviceToHost, g->s));
@@ -378,6 +421,7 @@ int dg_step(dg_grid* g, double dt, dg_status* st) {
         HIPCHK(hipEventElapsedTime(&ms, g->ev0, g->ev1));
         g->step_ms += ms;
         g->timed_steps++;
+        tm_flush(g);
     }
     const unsigned long long key = g->host_words[0];
     if (key != NO_ERR) {
@@ -448,10 +492,21 @@ int dg_timing(dg_grid* g, int enable, double* step_ms, long* steps) {
             HIPCHK(hipEventCreate(&g->ev0));
             HIPCHK(hipEventCreate(&g->ev1));
         }
+        HIPCHK(hipStreamSynchronize(g->s));
+        tm_flush(g);
         g->timing = enable != 0;
         g->step_ms = 0;
         g->timed_steps = 0;
+        for (int k = 0; k < K_COUNT; k++) g->kms[k] = 0, g->klaunch[k] = 0;
     }
+    return DG_OK;
+}
+
+int dg_kernel_time(dg_grid* g, int id, const char** name, double* ms, long* launches) {
+    if (!g || id < 0 || id >= K_COUNT) return DG_ERR_ARG;
+    if (name) *name = kKernelNames[id];
+    if (ms) *ms = g->kms[id];
+    if (launches) *launches = g->klaunch[id];
     return DG_OK;
 }
 
@@ -471,6 +526,321 @@ int dg_divb(dg_grid* g, double* out) {
     if (g->d.dim >= 2) dmin = std::min(dmin, g->d.dy);
     if (g->d.dim >= 3) dmin = std::min(dmin, g->d.dz);
     *out = bm > 0 ? dv * dmin / bm : 0.0;
+    return DG_OK;
+}
+
+}  // extern "C"
+
+// ======================================================= batched patches
+// Thousands of independent 2D MHD patches (CTU + CT), all advanced by one
+// persistent launch per step with per-patch dt.  The state of every patch is
+// in pools on the device; patch p is committed by flipping its parity word.
+struct dg_batch {
+    dg_desc d{};
+    Phys ph{};
+    int npatch = 0, bc_kind = 0;
+    std::vector<PatchDesc> P;
+    std::vector<int> order;                         // all patches, largest first
+    std::vector<int> cls_order[BATCH_CLASSES];      // per size class, largest first
+    PatchDesc* dP = nullptr;
+    int* d_order = nullptr;                         // [order | class 0 | class 1 | ...]
+    long cls_off[BATCH_CLASSES] = {};
+    double *W[2] = {nullptr, nullptr}, *B[2] = {nullptr, nullptr};
+    double *Fp = nullptr, *F = nullptr, *Bh = nullptr, *E0 = nullptr, *Eh = nullptr;
+    int* parity = nullptr;
+    double* dt_last = nullptr;
+    long *steps = nullptr, *fails = nullptr;
+    int* status = nullptr;
+    unsigned long long* err = nullptr;
+    unsigned* queue = nullptr;
+    double* stage = nullptr;                        // upload/download staging, largest patch
+    long cells = 0, faces = 0, zones = 0, max_nc = 0, max_nfc = 0;
+    int chunk[BATCH_CLASSES] = {8, 16, 32, 32};
+    int resident = 0;                               // workgroups that fit on the device at once
+    hipStream_t s = nullptr;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t gexec = nullptr;
+    int graph_steps = 0;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    std::string err_msg;
+};
+
+#define BCHK(x)                                                                \
+    do {                                                                       \
+        hipError_t e_ = (x);                                                   \
+        if (e_ != hipSuccess) {                                                \
+            if (b) b->err_msg = std::string(#x) + ": " + hipGetErrorString(e_); \
+            return DG_ERR_HIP;                                                 \
+        }                                                                      \
+    } while (0)
+
+static int batch_class(int nx, int ny) {
+    const int n = std::max(nx, ny);
+    return n < 64 ? 0 : n < 128 ? 1 : n < 192 ? 2 : 3;
+}
+
+static BatchArgs batch_args(const dg_batch* b, const int* order, int n, int commit, int init_only) {
+    BatchArgs a{};
+    a.P = b->dP; a.order = order; a.npatch = n;
+    a.W[0] = b->W[0]; a.W[1] = b->W[1]; a.B[0] = b->B[0]; a.B[1] = b->B[1];
+    a.Fp = b->Fp; a.F = b->F; a.Bh = b->Bh; a.E0 = b->E0; a.Eh = b->Eh;
+    a.parity = b->parity; a.dt_last = b->dt_last; a.steps = b->steps; a.fails = b->fails; a.status = b->status;
+    a.err = b->err; a.queue = b->queue;
+    a.ph = b->ph;
+    a.cfl_coeff = b->d.cfl_coeff; a.tol = b->d.timestep_tol > 0 ? b->d.timestep_tol : 1e-14;
+    a.cfl_mode = b->d.cfl_mode; a.bc_kind = b->bc_kind;
+    for (int c = 0; c < BATCH_CLASSES; c++) a.chunk[c] = b->chunk[c];
+    a.commit = commit; a.init_only = init_only;
+    return a;
+}
+
+static void batch_launch(dg_batch* b, const int* order, int n, int commit, int init_only) {
+    const int blocks = std::max(1, std::min(n, b->resident));
+    const BatchArgs a = batch_args(b, order, n, commit, init_only);
+    if (b->d.precision == DG_MIXED) p32::launch_batch(a, blocks, b->s);
+    else p64::launch_batch(a, blocks, b->s);
+}
+
+static void batch_drop_graph(dg_batch* b) {
+    if (b->gexec) (void)hipGraphExecDestroy(b->gexec);
+    if (b->graph) (void)hipGraphDestroy(b->graph);
+    b->gexec = nullptr; b->graph = nullptr; b->graph_steps = 0;
+}
+
+extern "C" {
+
+int dg_batch_create(const dg_desc* physics, int npatch, const int* nx, const int* ny, const double* dx, const double* dy,
+                    int bc_kind, dg_batch** out) {
+    if (!physics || !out || npatch < 1 || !nx || !ny || !dx || !dy) return DG_ERR_ARG;
+    if (!physics->mhd || !physics->ctu || !physics->muscl || (bc_kind != DG_BC_OUTFLOW && bc_kind != DG_BC_PERIODIC)) return DG_ERR_ARG;
+    for (int p = 0; p < npatch; p++)
+        if (nx[p] < 4 || ny[p] < 4 || !(dx[p] > 0) || !(dy[p] > 0)) return DG_ERR_ARG;
+    dg_batch* b = new dg_batch();
+    b->d = *physics;
+    b->d.dim = 2;
+    b->ph = make_phys(b->d);
+    b->npatch = npatch;
+    b->bc_kind = bc_kind;
+    b->P.resize(npatch);
+    const int G = 3;
+    for (int p = 0; p < npatch; p++) {
+        PatchDesc& q = b->P[p];
+        q.n[0] = nx[p]; q.n[1] = ny[p];
+        q.d[0] = dx[p]; q.d[1] = dy[p];
+        q.nc = (long)(nx[p] + 2 * G) * (ny[p] + 2 * G);
+        q.nfc = (long)(nx[p] + 1 + 2 * G) * (ny[p] + 1 + 2 * G);
+        q.oc = b->cells; q.of = b->faces;
+        q.cls = batch_class(nx[p], ny[p]);
+        b->cells += q.nc; b->faces += q.nfc;
+        b->zones += (long)nx[p] * ny[p];
+        b->max_nc = std::max(b->max_nc, q.nc);
+        b->max_nfc = std::max(b->max_nfc, q.nfc);
+    }
+    b->order.resize(npatch);
+    for (int p = 0; p < npatch; p++) b->order[p] = p;
+    std::stable_sort(b->order.begin(), b->order.end(), [&](int x, int y) { return b->P[x].nc > b->P[y].nc; });
+    for (int p : b->order) b->cls_order[b->P[p].cls].push_back(p);
+    std::vector<int> all(b->order);
+    for (int c = 0; c < BATCH_CLASSES; c++) {
+        b->cls_off[c] = (long)all.size();
+        all.insert(all.end(), b->cls_order[c].begin(), b->cls_order[c].end());
+    }
+    bool ok = hipStreamCreateWithFlags(&b->s, hipStreamNonBlocking) == hipSuccess;
+    auto alloc = [&](void** ptr, size_t bytes) { return ok && hipMalloc(ptr, bytes) == hipSuccess; };
+    const size_t cd = sizeof(double) * b->cells, fd = sizeof(double) * b->faces;
+    ok = alloc((void**)&b->W[0], 8 * cd) && alloc((void**)&b->W[1], 8 * cd) && alloc((void**)&b->B[0], 3 * fd) &&
+         alloc((void**)&b->B[1], 3 * fd) && alloc((void**)&b->Fp, 16 * cd) && alloc((void**)&b->F, 16 * cd) &&
+         alloc((void**)&b->Bh, 3 * fd) && alloc((void**)&b->E0, 3 * fd) && alloc((void**)&b->Eh, 3 * fd) &&
+         alloc((void**)&b->dP, sizeof(PatchDesc) * npatch) && alloc((void**)&b->d_order, sizeof(int) * all.size()) &&
+         alloc((void**)&b->parity, sizeof(int) * npatch) && alloc((void**)&b->dt_last, sizeof(double) * npatch) &&
+         alloc((void**)&b->steps, sizeof(long) * npatch) && alloc((void**)&b->fails, sizeof(long) * npatch) &&
+         alloc((void**)&b->status, sizeof(int) * npatch) && alloc((void**)&b->err, sizeof(unsigned long long) * npatch) &&
+         alloc((void**)&b->queue, 2 * sizeof(unsigned)) &&
+         alloc((void**)&b->stage, sizeof(double) * std::max(8 * b->max_nc, 3 * b->max_nfc));
+    ok = ok && hipMemcpyAsync(b->dP, b->P.data(), sizeof(PatchDesc) * npatch, hipMemcpyHostToDevice, b->s) == hipSuccess;
+    ok = ok && hipMemcpyAsync(b->d_order, all.data(), sizeof(int) * all.size(), hipMemcpyHostToDevice, b->s) == hipSuccess;
+    // set-up only (never inside a captured graph)
+    ok = ok && hipMemsetAsync(b->W[0], 0, 8 * cd, b->s) == hipSuccess && hipMemsetAsync(b->W[1], 0, 8 * cd, b->s) == hipSuccess;
+    ok = ok && hipMemsetAsync(b->B[0], 0, 3 * fd, b->s) == hipSuccess && hipMemsetAsync(b->B[1], 0, 3 * fd, b->s) == hipSuccess;
+    ok = ok && hipMemsetAsync(b->parity, 0, sizeof(int) * npatch, b->s) == hipSuccess;
+    ok = ok && hipMemsetAsync(b->dt_last, 0, sizeof(double) * npatch, b->s) == hipSuccess;
+    ok = ok && hipMemsetAsync(b->steps, 0, sizeof(long) * npatch, b->s) == hipSuccess;
+    ok = ok && hipMemsetAsync(b->fails, 0, sizeof(long) * npatch, b->s) == hipSuccess;
+    ok = ok && hipMemsetAsync(b->status, 0, sizeof(int) * npatch, b->s) == hipSuccess;
+    ok = ok && hipMemsetAsync(b->queue, 0, 2 * sizeof(unsigned), b->s) == hipSuccess;
+    ok = ok && hipEventCreate(&b->e0) == hipSuccess && hipEventCreate(&b->e1) == hipSuccess;
+    ok = ok && hipStreamSynchronize(b->s) == hipSuccess;
+    if (ok) {
+        int dev = 0, cus = 0;
+        ok = hipGetDevice(&dev) == hipSuccess && hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess;
+        b->resident = std::max(1, cus * (b->d.precision == DG_MIXED ? p32::batch_blocks_per_cu() : p64::batch_blocks_per_cu()));
+    }
+    if (!ok) {
+        dg_batch_destroy(b);
+        return DG_ERR_HIP;
+    }
+    *out = b;
+    return DG_OK;
+}
+
+int dg_batch_destroy(dg_batch* b) {
+    if (!b) return DG_OK;
+    int rc = DG_OK;
+    if (b->s && hipStreamSynchronize(b->s) != hipSuccess) rc = DG_ERR_HIP;
+    batch_drop_graph(b);
+    void* bufs[] = {b->W[0], b->W[1], b->B[0], b->B[1], b->Fp, b->F, b->Bh, b->E0, b->Eh, b->dP, b->d_order, b->parity,
+                    b->dt_last, b->steps, b->fails, b->status, b->err, b->queue, b->stage};
+    for (void* p : bufs)
+        if (p && hipFree(p) != hipSuccess) rc = DG_ERR_HIP;
+    if (b->e0 && hipEventDestroy(b->e0) != hipSuccess) rc = DG_ERR_HIP;
+    if (b->e1 && hipEventDestroy(b->e1) != hipSuccess) rc = DG_ERR_HIP;
+    if (b->s && hipStreamDestroy(b->s) != hipSuccess) rc = DG_ERR_HIP;
+    delete b;
+    return rc;
+}
+
+long dg_batch_zones(const dg_batch* b) { return b ? b->zones : -1; }
+int dg_batch_ghosts(const dg_batch* b) { return b ? 3 : -1; }
+const char* dg_batch_last_error(const dg_batch* b) { return b ? b->err_msg.c_str() : "null handle"; }
+void* dg_batch_stream(dg_batch* b) { return b ? (void*)b->s : nullptr; }
+
+int dg_batch_upload(dg_batch* b, int patch, const double* w, int stride, const double* B) {
+    if (!b || patch < 0 || patch >= b->npatch || !w || stride != 8 || !B) return DG_ERR_ARG;
+    const PatchDesc& q = b->P[patch];
+    const int par = 0;
+    BCHK(hipMemcpyAsync(b->stage, w, sizeof(double) * 8 * q.nc, hipMemcpyHostToDevice, b->s));
+    launch_aos_to_soa(b->stage, 8, 8, soa(b->W[par] + 8 * q.oc, q.nc), q.nc, b->s);
+    BCHK(hipMemcpyAsync(b->stage, B, sizeof(double) * 3 * q.nfc, hipMemcpyHostToDevice, b->s));
+    launch_aos_to_soa(b->stage, 3, 3, soa(b->B[par] + 3 * q.of, q.nfc), q.nfc, b->s);
+    BCHK(hipMemcpyAsync(b->parity + patch, &par, sizeof(int), hipMemcpyHostToDevice, b->s));
+    BCHK(hipStreamSynchronize(b->s));
+    return DG_OK;
+}
+
+int dg_batch_download(dg_batch* b, int patch, double* w, int stride, double* B) {
+    if (!b || patch < 0 || patch >= b->npatch || !w || stride != 8) return DG_ERR_ARG;
+    const PatchDesc& q = b->P[patch];
+    int par = 0;
+    BCHK(hipMemcpyAsync(&par, b->parity + patch, sizeof(int), hipMemcpyDeviceToHost, b->s));
+    BCHK(hipStreamSynchronize(b->s));
+    launch_soa_to_aos(soa(b->W[par] + 8 * q.oc, q.nc), 8, b->stage, 8, q.nc, b->s);
+    BCHK(hipMemcpyAsync(w, b->stage, sizeof(double) * 8 * q.nc, hipMemcpyDeviceToHost, b->s));
+    BCHK(hipStreamSynchronize(b->s));
+    if (B) {
+        launch_soa_to_aos(soa(b->B[par] + 3 * q.of, q.nfc), 3, b->stage, 3, q.nfc, b->s);
+        BCHK(hipMemcpyAsync(B, b->stage, sizeof(double) * 3 * q.nfc, hipMemcpyDeviceToHost, b->s));
+        BCHK(hipStreamSynchronize(b->s));
+    }
+    return DG_OK;
+}
+
+int dg_batch_init_B(dg_batch* b) {
+    if (!b) return DG_ERR_ARG;
+    batch_launch(b, b->d_order, b->npatch, 1, 1);
+    BCHK(hipGetLastError());
+    BCHK(hipStreamSynchronize(b->s));
+    return DG_OK;
+}
+
+int dg_batch_run(dg_batch* b, int nsteps, int use_graph, double* device_ms) {
+    if (!b || nsteps < 0) return DG_ERR_ARG;
+    if (nsteps == 0) return DG_OK;
+    if (use_graph && (!b->gexec || b->graph_steps != nsteps)) {
+        batch_drop_graph(b);
+        BCHK(hipStreamBeginCapture(b->s, hipStreamCaptureModeThreadLocal));
+        for (int k = 0; k < nsteps; k++) batch_launch(b, b->d_order, b->npatch, 1, 0);
+        hipGraph_t gr = nullptr;
+        BCHK(hipStreamEndCapture(b->s, &gr));
+        b->graph = gr;
+        BCHK(hipGraphInstantiate(&b->gexec, b->graph, nullptr, nullptr, 0));
+        b->graph_steps = nsteps;
+    }
+    if (device_ms) BCHK(hipEventRecord(b->e0, b->s));
+    if (use_graph) {
+        BCHK(hipGraphLaunch(b->gexec, b->s));
+    } else {
+        for (int k = 0; k < nsteps; k++) batch_launch(b, b->d_order, b->npatch, 1, 0);
+        BCHK(hipGetLastError());
+    }
+    if (device_ms) {
+        BCHK(hipEventRecord(b->e1, b->s));
+        BCHK(hipEventSynchronize(b->e1));
+        float ms = 0;
+        BCHK(hipEventElapsedTime(&ms, b->e0, b->e1));
+        *device_ms = ms;
+    }
+    return DG_OK;
+}
+
+int dg_batch_sync(dg_batch* b, dg_status* st) {
+    if (!b) return DG_ERR_ARG;
+    BCHK(hipStreamSynchronize(b->s));
+    std::vector<int> status(b->npatch);
+    BCHK(hipMemcpy(status.data(), b->status, sizeof(int) * b->npatch, hipMemcpyDeviceToHost));
+    for (int p = 0; p < b->npatch; p++)
+        if (status[p]) {
+            const char* msg = status[p] == DG_ERR_CFL ? "CFL timestep failed: max signal speed is zero"
+                                                       : "Timestep has fallen below minimum. Exiting";
+            set_status(st, status[p], p, 0, 0, msg);
+            b->err_msg = std::string(msg) + " (patch " + std::to_string(p) + ")";
+            return status[p];
+        }
+    set_status(st, DG_OK, 0, 0, 0, "");
+    return DG_OK;
+}
+
+int dg_batch_patch_stats(dg_batch* b, int patch, double* dt_last, long* steps, long* fails, int* status) {
+    if (!b || patch < 0 || patch >= b->npatch) return DG_ERR_ARG;
+    BCHK(hipStreamSynchronize(b->s));
+    if (dt_last) BCHK(hipMemcpy(dt_last, b->dt_last + patch, sizeof(double), hipMemcpyDeviceToHost));
+    if (steps) BCHK(hipMemcpy(steps, b->steps + patch, sizeof(long), hipMemcpyDeviceToHost));
+    if (fails) BCHK(hipMemcpy(fails, b->fails + patch, sizeof(long), hipMemcpyDeviceToHost));
+    if (status) BCHK(hipMemcpy(status, b->status + patch, sizeof(int), hipMemcpyDeviceToHost));
+    return DG_OK;
+}
+
+int dg_batch_set_tiles(dg_batch* b, const int* chunk, int n) {
+    if (!b || !chunk || n < 1 || n > BATCH_CLASSES) return DG_ERR_ARG;
+    for (int c = 0; c < n; c++)
+        if (chunk[c] < 1 || chunk[c] > 4096) return DG_ERR_ARG;
+    for (int c = 0; c < n; c++) b->chunk[c] = chunk[c];
+    batch_drop_graph(b);
+    return DG_OK;
+}
+
+int dg_batch_get_tiles(const dg_batch* b, int* chunk, int n) {
+    if (!b || !chunk || n < 1 || n > BATCH_CLASSES) return DG_ERR_ARG;
+    for (int c = 0; c < n; c++) chunk[c] = b->chunk[c];
+    return DG_OK;
+}
+
+int dg_batch_autotune(dg_batch* b, int reps, int* chunk_out, double* class_ms) {
+    if (!b || reps < 1) return DG_ERR_ARG;
+    static const int cand[] = {2, 4, 8, 16, 32, 64, 128};
+    for (int c = 0; c < BATCH_CLASSES; c++) {
+        const int n = (int)b->cls_order[c].size();
+        if (class_ms) class_ms[c] = 0;
+        if (!n) continue;
+        const int* ord = b->d_order + b->cls_off[c];
+        double best = 1e300;
+        int pick = b->chunk[c];
+        for (int k : cand) {
+            b->chunk[c] = k;
+            batch_launch(b, ord, n, 0, 0);  // warm, no commit
+            BCHK(hipEventRecord(b->e0, b->s));
+            for (int r = 0; r < reps; r++) batch_launch(b, ord, n, 0, 0);
+            BCHK(hipEventRecord(b->e1, b->s));
+            BCHK(hipEventSynchronize(b->e1));
+            float ms = 0;
+            BCHK(hipEventElapsedTime(&ms, b->e0, b->e1));
+            if (ms < best) { best = ms; pick = k; }
+        }
+        b->chunk[c] = pick;
+        if (class_ms) class_ms[c] = best / reps;
+    }
+    if (chunk_out)
+        for (int c = 0; c < BATCH_CLASSES; c++) chunk_out[c] = b->chunk[c];
+    batch_drop_graph(b);
     return DG_OK;
 }
 

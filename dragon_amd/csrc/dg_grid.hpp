@@ -24,7 +24,7 @@ struct Ext {
     __host__ __device__ long at(int i, int j, int k) const { return base + i * s[0] + j * s[1] + k * s[2]; }
 };
 
-inline Ext make_ext(int dim, int nx, int ny, int nz, int g) {
+__host__ __device__ inline Ext make_ext(int dim, int nx, int ny, int nz, int g) {
     Ext e;
     e.n[0] = nx; e.n[1] = dim >= 2 ? ny : 1; e.n[2] = dim >= 3 ? nz : 1;
     e.g = g;

@@ -8,9 +8,28 @@
 // Kernels are templated on the dimension D, on MHD, and on the arithmetic type
 // R of reconstruction and Riemann solves (double: drop-in; float: the mixed
 // form, whose state, accumulation, pressure recovery and CT stay double).
+#include <algorithm>
+#include <type_traits>
+
 #include "dg_kernels.hpp"
 
+#ifndef DG_PREC
+#define DG_PREC 64
+#endif
+#if DG_PREC == 32
+#define DG_NS p32
+#else
+#define DG_NS p64
+#endif
+
 namespace dg {
+namespace DG_NS {
+#if DG_PREC == 32
+using RT = float;   // arithmetic type of reconstruction and Riemann solves
+#else
+using RT = double;
+#endif
+void ctu_hydro3(const StepCtx& c, hipStream_t s);
 
 #define KB __launch_bounds__(256)
 
@@ -66,8 +85,7 @@ __device__ void copy_cell(const Soa& W, long dst, long src, int nf) {
     for (int f = 0; f < nf; f++) W(f, dst) = W(f, src);
 }
 
-KB __global__ void k_bc_cells(BCArgs a) {
-    const long t = tid();
+__device__ __forceinline__ void bc_cells_body(const BCArgs& a, long t) {
     const int n1 = a.hi[0] - a.lo[0], n2 = a.hi[1] - a.lo[1];
     if (t >= (long)n1 * n2) return;
     const int t0 = a.lo[0] + (int)(t / n2), t1 = a.lo[1] + (int)(t % n2);
@@ -100,12 +118,12 @@ KB __global__ void k_bc_cells(BCArgs a) {
         }
     }
 }
+KB __global__ void k_bc_cells(BCArgs a) { bc_cells_body(a, tid()); }
 
 __device__ __forceinline__ void fget(double* const B[3], long m, double v[3]) { v[0] = B[0][m]; v[1] = B[1][m]; v[2] = B[2][m]; }
 __device__ __forceinline__ void fset(double* const B[3], long m, const double v[3]) { B[0][m] = v[0]; B[1][m] = v[1]; B[2][m] = v[2]; }
 
-KB __global__ void k_bc_faces(BCArgs a) {
-    const long t = tid();
+__device__ __forceinline__ void bc_faces_body(const BCArgs& a, long t) {
     const int n1 = a.hi[0] - a.lo[0], n2 = a.hi[1] - a.lo[1];
     if (t >= (long)n1 * n2) return;
     const int t0 = a.lo[0] + (int)(t / n2), t1 = a.lo[1] + (int)(t % n2);
@@ -164,15 +182,15 @@ KB __global__ void k_bc_faces(BCArgs a) {
         }
     }
 }
+KB __global__ void k_bc_faces(BCArgs a) { bc_faces_body(a, tid()); }
 
 // ==================================================================== CFL
 // Max signal speed over cells [-1, n+1)^D (CFL.cpp:113-162); wave64
 // shuffles then one atomicMax per block on the (non-negative) FP64 bits.
 template <int D, bool MHD>
-KB __global__ void k_cfl(Soa W, Ext ce, Box box, Phys ph, int mode, double dx, double dy, double dz, unsigned long long* out) {
-    double m = 0;
-    int i, j, k;
-    if (box.point(tid(), i, j, k)) {
+__device__ __forceinline__ double cfl_speed_at(const Soa& W, const Ext& ce, int i, int j, int k, const Phys& ph, int mode, double dx,
+                                               double dy, double dz) {
+    {
         P<double> w = load_p<double>(W, ce.at(i, j, k), MHD);
         double a = MHD ? c_fast_max(w, ph) : cs(w, ph);
         double s = 0;
@@ -193,8 +211,16 @@ KB __global__ void k_cfl(Soa W, Ext ce, Box box, Phys ph, int mode, double dx, d
             if (D == 3 && dz > 1e-14) s += pow((dabs(w.vz) + a) / dz, p);
             s = pow(s, 1.0 / p);
         }
-        m = dmax(m, s);  // NaN never wins, as with std::max
+        return s;
     }
+}
+template <int D, bool MHD>
+KB __global__ void k_cfl(Soa W, Ext ce, Box box, Phys ph, int mode, double dx, double dy, double dz, unsigned long long* out) {
+    double m = 0;
+    int i, j, k;
+    const long cnt = box.count(), stride = (long)gridDim.x * blockDim.x;
+    for (long t = tid(); t < cnt; t += stride)  // grid-stride: few blocks, few atomics
+        if (box.point(t, i, j, k)) m = dmax(m, cfl_speed_at<D, MHD>(W, ce, i, j, k, ph, mode, dx, dy, dz));  // NaN never wins
     for (int off = 32; off > 0; off >>= 1) m = dmax(m, __shfl_xor(m, off));
     __shared__ double part[4];
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
@@ -439,14 +465,13 @@ __device__ __forceinline__ void face_flux(const SweepArgs& a, P<R> l, P<R> r, lo
 
 // Strided axis: one thread walks `chunk` consecutive faces.
 template <int D, bool MHD, class R, int AX, bool FINAL>
-KB __global__ void k_sweep_march(SweepArgs a) {
+__device__ __forceinline__ void sweep_march_body(const SweepArgs& a, long t) {
     constexpr int T0 = AX == 0 ? 1 : 0;            // slower transverse dim (3D only)
     constexpr int T1 = D == 3 ? (AX == 2 ? 1 : 2) : (AX == 0 ? 1 : 0);  // fastest transverse dim
     const long n1 = a.hi[T1] - a.lo[T1];
     const long n0 = D == 3 ? a.hi[T0] - a.lo[T0] : 1;
     const int len = a.hi[AX] - a.lo[AX];
     const int nch = (len + a.chunk - 1) / a.chunk;
-    const long t = tid();
     if (t >= (long)nch * n0 * n1) return;
     int idx[3] = {0, 0, 0};
     idx[T1] = a.lo[T1] + (int)(t % n1);
@@ -464,6 +489,8 @@ KB __global__ void k_sweep_march(SweepArgs a) {
         Rp = Rc;
     }
 }
+template <int D, bool MHD, class R, int AX, bool FINAL>
+KB __global__ void k_sweep_march(SweepArgs a) { sweep_march_body<D, MHD, R, AX, FINAL>(a, tid()); }
 
 template <class R> __device__ __forceinline__ P<R> shfl_up1(const P<R>& w) {
     return P<R>{__shfl_up(w.rho, 1), __shfl_up(w.p, 1), __shfl_up(w.vx, 1), __shfl_up(w.vy, 1),
@@ -473,11 +500,9 @@ template <class R> __device__ __forceinline__ P<R> shfl_up1(const P<R>& w) {
 // Contiguous axis: lane l of a wavefront rebuilds cell c0-1+l; lanes 1..63
 // take the left state from the lane below.
 template <int D, bool MHD, class R, int AX, bool FINAL>
-KB __global__ void k_sweep_wave(SweepArgs a) {
+__device__ __forceinline__ void sweep_wave_body(const SweepArgs& a, long wave, int lane) {
     constexpr int T0 = 0;                 // slowest transverse dim
     constexpr int T1 = D == 3 ? 1 : 0;    // faster transverse dim
-    const int lane = threadIdx.x & 63;
-    const long wave = tid() >> 6;
     const int len = a.hi[AX] - a.lo[AX];
     const int nseg = (len + 62) / 63;
     const long n1 = D == 3 ? a.hi[T1] - a.lo[T1] : 1;
@@ -496,6 +521,8 @@ KB __global__ void k_sweep_wave(SweepArgs a) {
     const P<R> Rl = shfl_up1(Rr);
     if (lane >= 1 && live) face_flux<MHD, R, AX>(a, Rl, L, a.ce.at(idx[0], idx[1], idx[2]));
 }
+template <int D, bool MHD, class R, int AX, bool FINAL>
+KB __global__ void k_sweep_wave(SweepArgs a) { sweep_wave_body<D, MHD, R, AX, FINAL>(a, tid() >> 6, threadIdx.x & 63); }
 
 // ===================================================== CTU hydro (2D/3D)
 // correctState (CTU.cpp:264-301): states along SAX corrected by the flux
@@ -561,10 +588,10 @@ struct ERef {
 // 2D corner EMF (Electric.cpp:24-35, 52-71) at edge (i,j).  With `fromW`,
 // Eref is computed from the cell states (predictor); otherwise it is read.
 template <bool FROM_W>
-KB __global__ void k_emf2(Soa FX, Soa FY, Soa W, ERef ref, Ext ce, Ext fe, Box box, int upwind, double* Ex, double* Ey,
-                          double* Ez) {
+__device__ __forceinline__ void emf2_body(const Soa& FX, const Soa& FY, const Soa& W, const ERef& ref, const Ext& ce, const Ext& fe,
+                                          const Box& box, int upwind, double* Ex, double* Ey, double* Ez, long t) {
     int i, j, k;
-    if (!box.point(tid(), i, j, k)) return;
+    if (!box.point(t, i, j, k)) return;
     auto f = [&](const Soa& F, int fld, int a, int b) { return F(fld, ce.at(a, b, 0)); };
     double ex = -f(FY, U_BZ, i, j);
     double ey = f(FX, U_BZ, i, j);
@@ -590,6 +617,11 @@ KB __global__ void k_emf2(Soa FX, Soa FY, Soa W, ERef ref, Ext ce, Ext fe, Box b
     }
     const long e = fe.at(i, j, 0);
     Ex[e] = ex; Ey[e] = ey; Ez[e] = ez;
+}
+template <bool FROM_W>
+KB __global__ void k_emf2(Soa FX, Soa FY, Soa W, ERef ref, Ext ce, Ext fe, Box box, int upwind, double* Ex, double* Ey,
+                          double* Ez) {
+    emf2_body<FROM_W>(FX, FY, W, ref, ce, fe, box, upwind, Ex, Ey, Ez, tid());
 }
 
 // 3D corner EMF (Electric.cpp:38-47, 74-100).
@@ -633,10 +665,9 @@ KB __global__ void k_emf3(Soa FX, Soa FY, Soa FZ, Soa W, ERef ref, Ext ce, Ext f
 // Faraday (B_fields.cpp:21-66) fused with the clone: Bout = Bin everywhere,
 // minus dt*curl(E) on the faces of the reference's ranges (ghost width gg).
 template <int D>
-KB __global__ void k_faraday(const double* Ex, const double* Ey, const double* Ez, const double* Bix, const double* Biy,
-                             const double* Biz, double* Box_, double* Boy, double* Boz, Ext fe, int gg, double dt_dx, double dt_dy,
-                             double dt_dz) {
-    const long m = tid();
+__device__ __forceinline__ void faraday_body(const double* Ex, const double* Ey, const double* Ez, const double* Bix, const double* Biy,
+                                             const double* Biz, double* Box_, double* Boy, double* Boz, const Ext& fe, int gg, double dt_dx,
+                                             double dt_dy, double dt_dz, long m) {
     if (m >= fe.size) return;
     double bx = Bix[m], by = Biy[m], bz = Biz[m];
     // recover (i,j,k) of m
@@ -660,6 +691,12 @@ KB __global__ void k_faraday(const double* Ex, const double* Ey, const double* E
         if (ixl && jyl && kz) bz -= (Ey[m + fe.s[0]] - Ey[m]) * dt_dx - (Ex[m + fe.s[1]] - Ex[m]) * dt_dy;
     }
     Box_[m] = bx; Boy[m] = by; Boz[m] = bz;
+}
+template <int D>
+KB __global__ void k_faraday(const double* Ex, const double* Ey, const double* Ez, const double* Bix, const double* Biy,
+                             const double* Biz, double* Box_, double* Boy, double* Boz, Ext fe, int gg, double dt_dx, double dt_dy,
+                             double dt_dz) {
+    faraday_body<D>(Ex, Ey, Ez, Bix, Biy, Biz, Box_, Boy, Boz, fe, gg, dt_dx, dt_dy, dt_dz, tid());
 }
 
 // computeBodyField + shouldProtectThermal (B_fields.cpp:70-114, production
@@ -692,11 +729,11 @@ __device__ __forceinline__ void cell_B(const double* Bx, const double* By, const
 // at dt/2 over cells [-1, n+1), body fields from the half-step faces, then
 // Ehalf = cross(B, v) over cells [-1, n].
 template <int D>
-KB __global__ void k_whalf(Soa W, Soa FX, Soa FY, Soa FZ, const double* Bhx, const double* Bhy, const double* Bhz, Ext ce,
-                           Ext fe, Box box, double hx, double hy, double hz, Phys ph, double* Ehx, double* Ehy, double* Ehz,
-                           unsigned long long* err) {
+__device__ __forceinline__ void whalf_body(const Soa& W, const Soa& FX, const Soa& FY, const Soa& FZ, const double* Bhx, const double* Bhy,
+                                           const double* Bhz, const Ext& ce, const Ext& fe, const Box& box, double hx, double hy, double hz,
+                                           const Phys& ph, double* Ehx, double* Ehy, double* Ehz, unsigned long long* err, long t) {
     int i, j, k;
-    if (!box.point(tid(), i, j, k)) return;
+    if (!box.point(t, i, j, k)) return;
     const long m = ce.at(i, j, k);
     P<double> w = load_p<double>(W, m, true);
     U<double> u = cons<true>(w, ph);
@@ -712,6 +749,12 @@ KB __global__ void k_whalf(Soa W, Soa FX, Soa FY, Soa FZ, const double* Bhx, con
     cross(w.Bx, w.By, w.Bz, w.vx, w.vy, w.vz, ex, ey, ez);
     const long e = fe.at(i, j, k);
     Ehx[e] = ex; Ehy[e] = ey; Ehz[e] = ez;
+}
+template <int D>
+KB __global__ void k_whalf(Soa W, Soa FX, Soa FY, Soa FZ, const double* Bhx, const double* Bhy, const double* Bhz, Ext ce,
+                           Ext fe, Box box, double hx, double hy, double hz, Phys ph, double* Ehx, double* Ehy, double* Ehz,
+                           unsigned long long* err) {
+    whalf_body<D>(W, FX, FY, FZ, Bhx, Bhy, Bhz, ce, fe, box, hx, hy, hz, ph, Ehx, Ehy, Ehz, err, tid());
 }
 
 // ctu_sweep_MHD corrections (CTU.cpp:65-129 3D, 162-202 2D), in place.
@@ -788,9 +831,9 @@ KB __global__ void k_ctu_mhd(Soa W, const double* Bx, const double* By, const do
 // next state: interior cells updated, ghost cells copied (the reference
 // commits interior cells only, so the old ghosts remain).
 template <int D, bool MHD>
-KB __global__ void k_apply(Soa W, Soa Wn, Soa FX, Soa FY, Soa FZ, const double* Bnx, const double* Bny, const double* Bnz, Ext ce,
-                           Ext fe, double dt_dx, double dt_dy, double dt_dz, Phys ph, unsigned long long* err) {
-    const long m = tid();
+__device__ __forceinline__ void apply_body(const Soa& W, const Soa& Wn, const Soa& FX, const Soa& FY, const Soa& FZ, const double* Bnx,
+                                           const double* Bny, const double* Bnz, const Ext& ce, const Ext& fe, double dt_dx, double dt_dy,
+                                           double dt_dz, const Phys& ph, unsigned long long* err, long m) {
     if (m >= ce.size) return;
     const int nf = MHD ? 8 : 5;
     long r = m;
@@ -818,6 +861,11 @@ KB __global__ void k_apply(Soa W, Soa Wn, Soa FX, Soa FY, Soa FZ, const double* 
     }
     if (!physical<MHD>(w)) report(err, ST_UNPHYSICAL, (unsigned long long)m, E_UNPHYSICAL);
     store_p<double>(Wn, m, w, MHD);
+}
+template <int D, bool MHD>
+KB __global__ void k_apply(Soa W, Soa Wn, Soa FX, Soa FY, Soa FZ, const double* Bnx, const double* Bny, const double* Bnz, Ext ce,
+                           Ext fe, double dt_dx, double dt_dy, double dt_dz, Phys ph, unsigned long long* err) {
+    apply_body<D, MHD>(W, Wn, FX, FY, FZ, Bnx, Bny, Bnz, ce, fe, dt_dx, dt_dy, dt_dz, ph, err, tid());
 }
 
 // 1D Godunov::sweep update (Splitting.cpp:67-95): cells [-g+1, n+g-2] take
@@ -934,8 +982,8 @@ template <int D, bool MHD>
 static void cfl_t(const StepCtx& c, unsigned long long* out, hipStream_t s) {
     const int n1 = c.ce.g < 1 ? c.ce.g : 1;
     Box b = make_box(D, -n1, c.ce.n[0] + n1, -n1, c.ce.n[1] + n1, -n1, c.ce.n[2] + n1);
-    hipLaunchKernelGGL((k_cfl<D, MHD>), dim3(blocks(b.count())), dim3(256), 0, s, c.W, c.ce, b, c.ph, c.cfl_mode, c.d[0], c.d[1],
-                       c.d[2], out);
+    const unsigned nb = std::min(blocks(b.count()), 8192u);
+    hipLaunchKernelGGL((k_cfl<D, MHD>), dim3(nb), dim3(256), 0, s, c.W, c.ce, b, c.ph, c.cfl_mode, c.d[0], c.d[1], c.d[2], out);
 }
 void launch_cfl(const StepCtx& c, unsigned long long* out, hipStream_t s) {
     if (c.dim == 1) c.mhd ? cfl_t<1, true>(c, out, s) : cfl_t<1, false>(c, out, s);
@@ -1017,6 +1065,7 @@ static void step_fused(const StepCtx& c, hipStream_t s) {
         for (int ax = 0; ax < D; ax++) {
             Box b = MHD ? face_box(D, ce, ax, -1, 1, -2, 2) : face_box(D, ce, ax, 0, 0, -1, 1);
             sweep<D, MHD, R, false>(c, ax, b, c.Fp[ax], ST_PRED_X + ax, s);
+            mark(c, K_PRED_X + ax);
         }
         if constexpr (MHD) {
             ERef none{nullptr, nullptr, nullptr};
@@ -1027,18 +1076,22 @@ static void step_fused(const StepCtx& c, hipStream_t s) {
             else
                 hipLaunchKernelGGL((k_emf3<true>), dim3(blocks(eb.count())), dim3(256), 0, s, c.Fp[0], c.Fp[1], c.Fp[2], c.W, none, ce,
                                    c.fe, eb, 1, c.E0[0], c.E0[1], c.E0[2]);
+            mark(c, K_EMF_PRED);
             hipLaunchKernelGGL((k_faraday<D>), dim3(blocks(c.fe.size)), dim3(256), 0, s, c.E0[0], c.E0[1], c.E0[2], c.B[0], c.B[1],
                                c.B[2], c.Bh[0], c.Bh[1], c.Bh[2], c.fe, 1, 0.5 * dt_d[0], 0.5 * dt_d[1], 0.5 * dt_d[2]);
+            mark(c, K_FARADAY_HALF);
             Box cb = make_box(D, -1, ce.n[0] + 1, -1, ce.n[1] + 1, -1, ce.n[2] + 1);
             hipLaunchKernelGGL((k_whalf<D>), dim3(blocks(cb.count())), dim3(256), 0, s, c.W, c.Fp[0], c.Fp[1], c.Fp[2], c.Bh[0],
                                c.Bh[1], c.Bh[2], ce, c.fe, cb, 0.5 * dt_d[0], 0.5 * dt_d[1], 0.5 * dt_d[2], c.ph, c.Eh[0], c.Eh[1],
                                c.Eh[2], c.err);
+            mark(c, K_WHALF);
         }
     }
     for (int ax = 0; ax < D; ax++) {
         Box b = MHD ? face_box(D, ce, ax, 0, 0, -1, 1) : face_box(D, ce, ax, 0, 0, 0, 0);
         if (c.ctu) sweep<D, MHD, R, true>(c, ax, b, c.F[ax], ST_FLUX_X + ax, s);
         else sweep<D, MHD, R, false>(c, ax, b, c.F[ax], ST_FLUX_X + ax, s);
+        mark(c, K_FLUX_X + ax);
     }
     if constexpr (MHD) {
         Box eb = make_box(D, 0, ce.n[0] + 1, 0, ce.n[1] + 1, 0, ce.n[2] + 1);
@@ -1049,11 +1102,14 @@ static void step_fused(const StepCtx& c, hipStream_t s) {
         else
             hipLaunchKernelGGL((k_emf3<false>), dim3(blocks(eb.count())), dim3(256), 0, s, c.F[0], c.F[1], c.F[2], c.W, eh, ce, c.fe, eb,
                                c.ctu, c.E0[0], c.E0[1], c.E0[2]);
+        mark(c, K_EMF);
         hipLaunchKernelGGL((k_faraday<D>), dim3(blocks(c.fe.size)), dim3(256), 0, s, c.E0[0], c.E0[1], c.E0[2], c.B[0], c.B[1], c.B[2],
                            c.Bn[0], c.Bn[1], c.Bn[2], c.fe, 0, dt_d[0], dt_d[1], dt_d[2]);
+        mark(c, K_FARADAY);
     }
     hipLaunchKernelGGL((k_apply<D, MHD>), dim3(blocks(ce.size)), dim3(256), 0, s, c.W, c.Wn, c.F[0], c.F[1], c.F[2], c.Bn[0], c.Bn[1],
                        c.Bn[2], ce, c.fe, dt_d[0], dt_d[1], dt_d[2], c.ph, c.err);
+    mark(c, K_APPLY);
 }
 
 template <int D, bool MHD, class R>
@@ -1072,20 +1128,25 @@ static void step_t(const StepCtx& c, hipStream_t s) {
         Box all = make_box(D, -g, ce.n[0] + g, -g, ce.n[1] + g, -g, ce.n[2] + g);
         hipLaunchKernelGGL((k_half<D, MHD, R>), dim3(blocks(all.count())), dim3(256), 0, s, c.W, c.B[0], c.B[1], c.B[2], ce, c.fe,
                            c.HS[0], c.HS[1], c.HS[2], c.HS[3], c.HS[4], c.HS[5], dt_d[0], dt_d[1], dt_d[2], c.ph);
+        mark(c, K_HALF);
     }
     if (D == 1) {
         Box fb = make_box(1, -g + 1, ce.n[0] + g, 0, 1, 0, 1);
         flux_ax<D, MHD, R>(c, 0, c.HS[0], c.HS[1], c.F[0], fb, dt_d[0], ST_FLUX_X, s);
+        mark(c, K_FLUX_X);
         if constexpr (D == 1)
             hipLaunchKernelGGL((k_apply1<MHD>), dim3(blocks(ce.size)), dim3(256), 0, s, c.W, c.Wn, c.F[0], ce, dt_d[0], c.ph, c.err);
+        mark(c, K_APPLY);
         return;
     }
     if (c.ctu) {
         if constexpr (MHD) {
             // ctu_sweep_MHD predictor fluxes on [-1, n+1] x [-2, n+2)
-            for (int ax = 0; ax < D; ax++)
+            for (int ax = 0; ax < D; ax++) {
                 flux_ax<D, MHD, R>(c, ax, c.HS[2 * ax], c.HS[2 * ax + 1], c.F[ax], face_box(D, ce, ax, -1, 1, -2, 2), dt_d[ax],
                                    ST_PRED_X + ax, s);
+                mark(c, K_PRED_X + ax);
+            }
             ERef none{nullptr, nullptr, nullptr};
             Box eb = make_box(D, -1, ce.n[0] + 2, -1, ce.n[1] + 2, -1, ce.n[2] + 2);
             if constexpr (D == 2)
@@ -1094,38 +1155,51 @@ static void step_t(const StepCtx& c, hipStream_t s) {
             else
                 hipLaunchKernelGGL((k_emf3<true>), dim3(blocks(eb.count())), dim3(256), 0, s, c.F[0], c.F[1], c.F[2], c.W, none, ce, c.fe,
                                    eb, 1, c.E0[0], c.E0[1], c.E0[2]);
+            mark(c, K_EMF_PRED);
             hipLaunchKernelGGL((k_faraday<D>), dim3(blocks(c.fe.size)), dim3(256), 0, s, c.E0[0], c.E0[1], c.E0[2], c.B[0], c.B[1],
                                c.B[2], c.Bh[0], c.Bh[1], c.Bh[2], c.fe, 1, 0.5 * dt_d[0], 0.5 * dt_d[1], 0.5 * dt_d[2]);
+            mark(c, K_FARADAY_HALF);
             Box cb = make_box(D, -1, ce.n[0] + 1, -1, ce.n[1] + 1, -1, ce.n[2] + 1);
             hipLaunchKernelGGL((k_whalf<D>), dim3(blocks(cb.count())), dim3(256), 0, s, c.W, c.F[0], c.F[1], c.F[2], c.Bh[0], c.Bh[1],
                                c.Bh[2], ce, c.fe, cb, 0.5 * dt_d[0], 0.5 * dt_d[1], 0.5 * dt_d[2], c.ph, c.Eh[0], c.Eh[1], c.Eh[2],
                                c.err);
+            mark(c, K_WHALF);
             hipLaunchKernelGGL((k_ctu_mhd<D>), dim3(blocks(cb.count())), dim3(256), 0, s, c.W, c.B[0], c.B[1], c.B[2], c.Bh[0], c.Bh[1],
                                c.Bh[2], c.E0[0], c.E0[1], c.E0[2], c.F[0], c.F[1], c.F[2], c.HS[0], c.HS[1], c.HS[2], c.HS[3],
                                c.HS[4], c.HS[5], ce, c.fe, cb, dt_d[0], dt_d[1], dt_d[2], c.ph);
+            mark(c, K_CTU);
         } else if constexpr (D == 2) {
             // ctu_sweep_hydro 2D (CTU.cpp:206-218)
             flux_ax<D, MHD, R>(c, 0, c.HS[0], c.HS[1], c.F[0], face_box(D, ce, 0, 0, 0, -1, 1), dt_d[0], ST_PRED_X, s);
+            mark(c, K_PRED_X);
             flux_ax<D, MHD, R>(c, 1, c.HS[2], c.HS[3], c.F[1], face_box(D, ce, 1, 0, 0, -1, 1), dt_d[1], ST_PRED_Y, s);
+            mark(c, K_PRED_Y);
             Box bx = make_box(2, -1, ce.n[0] + 1, 0, ce.n[1]);
             hipLaunchKernelGGL((k_correct<D, R>), dim3(blocks(bx.count())), dim3(256), 0, s, c.HS[0], c.HS[1], c.F[1], ce, bx, 1,
                                0.5 * dt_d[1], c.ph);
+            mark(c, K_CTU);
             Box by = make_box(2, 0, ce.n[0], -1, ce.n[1] + 1);
             hipLaunchKernelGGL((k_correct<D, R>), dim3(blocks(by.count())), dim3(256), 0, s, c.HS[2], c.HS[3], c.F[0], ce, by, 0,
                                0.5 * dt_d[0], c.ph);
+            mark(c, K_CTU);
         } else {
             ctu_hydro3(c, s);
+            mark(c, K_CTU);
         }
     }
     // final fluxes
     if (MHD) {
-        for (int ax = 0; ax < D; ax++)
+        for (int ax = 0; ax < D; ax++) {
             flux_ax<D, MHD, R>(c, ax, c.HS[2 * ax], c.HS[2 * ax + 1], c.F[ax], face_box(D, ce, ax, 0, 0, -1, 1), dt_d[ax],
                                ST_FLUX_X + ax, s);
+            mark(c, K_FLUX_X + ax);
+        }
     } else {
-        for (int ax = 0; ax < D; ax++)
+        for (int ax = 0; ax < D; ax++) {
             flux_ax<D, MHD, R>(c, ax, c.HS[2 * ax], c.HS[2 * ax + 1], c.F[ax], face_box(D, ce, ax, 0, 0, 0, 0), dt_d[ax],
                                ST_FLUX_X + ax, s);
+            mark(c, K_FLUX_X + ax);
+        }
     }
     if constexpr (MHD) {
         Box eb = make_box(D, 0, ce.n[0] + 1, 0, ce.n[1] + 1, 0, ce.n[2] + 1);
@@ -1136,11 +1210,14 @@ static void step_t(const StepCtx& c, hipStream_t s) {
         else
             hipLaunchKernelGGL((k_emf3<false>), dim3(blocks(eb.count())), dim3(256), 0, s, c.F[0], c.F[1], c.F[2], c.W, eh, ce, c.fe, eb,
                                c.ctu, c.E0[0], c.E0[1], c.E0[2]);
+        mark(c, K_EMF);
         hipLaunchKernelGGL((k_faraday<D>), dim3(blocks(c.fe.size)), dim3(256), 0, s, c.E0[0], c.E0[1], c.E0[2], c.B[0], c.B[1], c.B[2],
                            c.Bn[0], c.Bn[1], c.Bn[2], c.fe, 0, dt_d[0], dt_d[1], dt_d[2]);
+        mark(c, K_FARADAY);
     }
     hipLaunchKernelGGL((k_apply<D, MHD>), dim3(blocks(ce.size)), dim3(256), 0, s, c.W, c.Wn, c.F[0], c.F[1], c.F[2], c.Bn[0], c.Bn[1],
                        c.Bn[2], ce, c.fe, dt_d[0], dt_d[1], dt_d[2], c.ph, c.err);
+    mark(c, K_APPLY);
 }
 
 // 3D hydro CTU, 12 solves (CTU.cpp:222-261): six corner-flux passes with
@@ -1190,18 +1267,225 @@ void ctu_hydro3(const StepCtx& c, hipStream_t s) {
     corr(zL, zR, FYx, 0.5 * dy, 2, 1);
 }
 
-void launch_step(const StepCtx& c, hipStream_t s) {
-    const bool f32 = c.mixed;
-    if (c.dim == 1) {
-        if (c.mhd) f32 ? step_t<1, true, float>(c, s) : step_t<1, true, double>(c, s);
-        else f32 ? step_t<1, false, float>(c, s) : step_t<1, false, double>(c, s);
-    } else if (c.dim == 2) {
-        if (c.mhd) f32 ? step_t<2, true, float>(c, s) : step_t<2, true, double>(c, s);
-        else f32 ? step_t<2, false, float>(c, s) : step_t<2, false, double>(c, s);
-    } else {
-        if (c.mhd) f32 ? step_t<3, true, float>(c, s) : step_t<3, true, double>(c, s);
-        else f32 ? step_t<3, false, float>(c, s) : step_t<3, false, double>(c, s);
+// ===================================================== batched patches
+// One workgroup advances one 2D MHD patch through the whole CTU + CT step:
+// boundary passes, CFL reduction, predictor sweeps, corner EMF, half-step
+// faces and state, corrector sweeps, EMF, Faraday, apply.  These are the
+// single-grid kernels' bodies run as workgroup-strided loops with a barrier
+// between stages.  A rejected attempt halves the patch's dt and repeats on
+// the device, as Grid::advance does per grid (CFL.cpp:151-213).  Workgroups
+// take patches from a queue, largest first; the last workgroup to finish
+// resets the queue, so one launch is one step of every patch and the launch
+// can be replayed from a graph.
+__device__ __forceinline__ void bsync() { __syncthreads(); }
+
+template <class R>
+__device__ void batch_patch(const BatchArgs& A, int p, double* s_red, unsigned long long* s_key) {
+    const PatchDesc P = A.P[p];
+    constexpr int G = 3;  // validGhosts for MHD + CTU
+    const Ext ce = make_ext(2, P.n[0], P.n[1], 1, G);
+    const Ext fe = make_ext(2, P.n[0] + 1, P.n[1] + 1, 1, G);
+    const int par = A.parity[p];
+    const Soa W{A.W[par] + 8 * P.oc, P.nc}, Wn{A.W[1 - par] + 8 * P.oc, P.nc};
+    double *B[3], *Bn[3], *Bh[3], *E0[3], *Eh[3];
+    for (int c = 0; c < 3; c++) {
+        B[c] = A.B[par] + 3 * P.of + c * P.nfc;
+        Bn[c] = A.B[1 - par] + 3 * P.of + c * P.nfc;
+        Bh[c] = A.Bh + 3 * P.of + c * P.nfc;
+        E0[c] = A.E0 + 3 * P.of + c * P.nfc;
+        Eh[c] = A.Eh + 3 * P.of + c * P.nfc;
+    }
+    Soa Fp[3], F[3];
+    for (int ax = 0; ax < 2; ax++) {
+        Fp[ax] = Soa{A.Fp + 16 * P.oc + ax * 8 * P.nc, P.nc};
+        F[ax] = Soa{A.F + 16 * P.oc + ax * 8 * P.nc, P.nc};
+    }
+    Fp[2] = Fp[0];  // no z axis in 2D; never read
+    F[2] = F[0];
+    unsigned long long* err = A.err + p;
+    const int tx = threadIdx.x, nt = blockDim.x;
+
+    // boundary: cell passes by axis and side, then face passes (the order of
+    // one list element's apply, corner ghosts on)
+    {
+        BCArgs b{};
+        b.W = W;
+        for (int c = 0; c < 3; c++) b.B[c] = B[c];
+        b.ce = ce; b.fe = fe; b.dim = 2; b.nf = 8; b.kind = A.bc_kind; b.corners = 1; b.gated = 0;
+        const int s0 = A.bc_kind == 1 ? 2 : 0, s1 = A.bc_kind == 1 ? 2 : 1;
+        for (int pass = 0; pass < 2; pass++)
+            for (int ax = 0; ax < 2; ax++) {
+                b.axis = ax;
+                b.lo[0] = -G; b.hi[0] = ce.n[ax == 0 ? 1 : 0] + G + pass;
+                b.lo[1] = 0; b.hi[1] = 1;
+                const long cnt = b.hi[0] - b.lo[0];
+                for (int side = s0; side <= s1; side++) {
+                    b.side = side;
+                    for (long t = tx; t < cnt; t += nt) {
+                        if (pass == 0) bc_cells_body(b, t);
+                        else bc_faces_body(b, t);
+                    }
+                    bsync();
+                }
+            }
+    }
+    if (A.init_only) {  // initialize_B_fields: cell B = face average everywhere, p held
+        for (long m = tx; m < ce.size; m += nt) {
+            const int i = (int)(m / (ce.n[1] + 2 * G)) - G, j = (int)(m % (ce.n[1] + 2 * G)) - G;
+            double x, y, z;
+            cell_B(B[0], B[1], B[2], fe, 2, i, j, 0, x, y, z);
+            W(F_BX, m) = x; W(F_BY, m) = y; W(F_BZ, m) = z;
+        }
+        return;
+    }
+
+    // CFL: max signal speed over cells [-1, n+1)^2 (NaN never wins)
+    double dt;
+    {
+        double m = 0;
+        const long ny2 = ce.n[1] + 2, cnt = (long)(ce.n[0] + 2) * ny2;
+        for (long t = tx; t < cnt; t += nt)
+            m = dmax(m, cfl_speed_at<2, true>(W, ce, (int)(t / ny2) - 1, (int)(t % ny2) - 1, 0, A.ph, A.cfl_mode, P.d[0], P.d[1], 1.0));
+        for (int off = 32; off > 0; off >>= 1) m = dmax(m, __shfl_xor(m, off));
+        if ((tx & 63) == 0) s_red[tx >> 6] = m;
+        bsync();
+        m = s_red[0];
+        for (int q = 1; q < (nt >> 6); q++) m = dmax(m, s_red[q]);
+        bsync();
+        if (!(m > 0)) {
+            if (tx == 0 && A.commit) A.status[p] = BATCH_ERR_CFL;
+            return;
+        }
+        dt = A.cfl_coeff / m;
+    }
+
+    const int chunk = A.chunk[P.cls];
+    long nfail = 0;
+    while (true) {
+        if (tx == 0) atomicExch(err, ~0ull);
+        __threadfence();
+        bsync();
+        const double dt_d[3] = {dt / P.d[0], dt / P.d[1], dt / 1.0};
+        SweepArgs s{};
+        s.W = W;
+        for (int q = 0; q < 3; q++) {
+            s.B[q] = B[q]; s.Bh[q] = Bh[q]; s.E0[q] = E0[q]; s.Fp[q] = Fp[q]; s.dt_d[q] = dt_d[q];
+        }
+        s.ce = ce; s.fe = fe; s.ph = A.ph; s.err = err;
+        auto box = [&](int ax, int alo, int ahi, int tlo, int thi) {
+            const int t = ax == 0 ? 1 : 0;
+            s.lo[ax] = alo; s.hi[ax] = ahi; s.lo[t] = tlo; s.hi[t] = thi; s.lo[2] = 0; s.hi[2] = 1;
+        };
+        auto march = [&](auto fin) {  // x faces, strided axis: chunk faces per thread
+            constexpr bool FIN = decltype(fin)::value;
+            const int len = s.hi[0] - s.lo[0];
+            s.chunk = chunk < len ? chunk : len;
+            const long items = (long)((len + s.chunk - 1) / s.chunk) * (s.hi[1] - s.lo[1]);
+            for (long t = tx; t < items; t += nt) sweep_march_body<2, true, R, 0, FIN>(s, t);
+            bsync();
+        };
+        auto wave = [&](auto fin) {  // y faces, contiguous axis: 63 faces per wavefront
+            constexpr bool FIN = decltype(fin)::value;
+            s.chunk = 63;
+            const long waves = (long)((s.hi[1] - s.lo[1] + 62) / 63) * (s.hi[0] - s.lo[0]);
+            for (long w = tx >> 6; w < waves; w += nt >> 6) sweep_wave_body<2, true, R, 1, FIN>(s, w, tx & 63);
+            bsync();
+        };
+        // predictor fluxes on [-1, n+1] x [-2, n+2)
+        box(0, -1, ce.n[0] + 2, -2, ce.n[1] + 2); s.Fo = Fp[0]; s.stage = ST_PRED_X;
+        march(std::false_type{});
+        box(1, -1, ce.n[1] + 2, -2, ce.n[0] + 2); s.Fo = Fp[1]; s.stage = ST_PRED_Y;
+        wave(std::false_type{});
+        {
+            const ERef none{nullptr, nullptr, nullptr};
+            const Box eb = make_box(2, -1, ce.n[0] + 2, -1, ce.n[1] + 2);
+            for (long t = tx; t < eb.count(); t += nt) emf2_body<true>(Fp[0], Fp[1], W, none, ce, fe, eb, 1, E0[0], E0[1], E0[2], t);
+            bsync();
+            for (long m = tx; m < fe.size; m += nt)
+                faraday_body<2>(E0[0], E0[1], E0[2], B[0], B[1], B[2], Bh[0], Bh[1], Bh[2], fe, 1, 0.5 * dt_d[0], 0.5 * dt_d[1],
+                                0.5 * dt_d[2], m);
+            bsync();
+            const Box cb = make_box(2, -1, ce.n[0] + 1, -1, ce.n[1] + 1);
+            for (long t = tx; t < cb.count(); t += nt)
+                whalf_body<2>(W, Fp[0], Fp[1], Fp[2], Bh[0], Bh[1], Bh[2], ce, fe, cb, 0.5 * dt_d[0], 0.5 * dt_d[1], 0.5 * dt_d[2], A.ph,
+                              Eh[0], Eh[1], Eh[2], err, t);
+            bsync();
+        }
+        // corrector fluxes on [0, n] x [-1, n+1), CTU correction inside
+        box(0, 0, ce.n[0] + 1, -1, ce.n[1] + 1); s.Fo = F[0]; s.stage = ST_FLUX_X;
+        march(std::true_type{});
+        box(1, 0, ce.n[1] + 1, -1, ce.n[0] + 1); s.Fo = F[1]; s.stage = ST_FLUX_Y;
+        wave(std::true_type{});
+        {
+            const ERef eh{Eh[0], Eh[1], Eh[2]};
+            const Box eb = make_box(2, 0, ce.n[0] + 1, 0, ce.n[1] + 1);
+            for (long t = tx; t < eb.count(); t += nt) emf2_body<false>(F[0], F[1], W, eh, ce, fe, eb, 1, E0[0], E0[1], E0[2], t);
+            bsync();
+            for (long m = tx; m < fe.size; m += nt)
+                faraday_body<2>(E0[0], E0[1], E0[2], B[0], B[1], B[2], Bn[0], Bn[1], Bn[2], fe, 0, dt_d[0], dt_d[1], dt_d[2], m);
+            bsync();
+            for (long m = tx; m < ce.size; m += nt)
+                apply_body<2, true>(W, Wn, F[0], F[1], F[2], Bn[0], Bn[1], Bn[2], ce, fe, dt_d[0], dt_d[1], dt_d[2], A.ph, err, m);
+        }
+        __threadfence();
+        bsync();
+        if (tx == 0) *s_key = atomicOr(err, 0ull);
+        bsync();
+        const unsigned long long key = *s_key;
+        if (key == ~0ull) break;
+        nfail++;
+        dt *= 0.5;
+        if (dt < A.tol) {
+            if (tx == 0 && A.commit) { A.status[p] = BATCH_ERR_DT; A.fails[p] += nfail; }
+            return;
+        }
+    }
+    if (tx == 0 && A.commit) {
+        A.parity[p] = 1 - par;
+        A.dt_last[p] = dt;
+        A.steps[p] += 1;
+        A.fails[p] += nfail;
+        A.status[p] = 0;
     }
 }
 
+template <class R>
+__global__ __launch_bounds__(BATCH_THREADS) void k_batch(BatchArgs A) {
+    __shared__ double s_red[BATCH_THREADS / 64];
+    __shared__ unsigned long long s_key;
+    __shared__ int s_next;
+    while (true) {
+        if (threadIdx.x == 0) s_next = (int)atomicAdd(&A.queue[0], 1u);
+        __syncthreads();
+        const int q = s_next;
+        __syncthreads();
+        if (q >= A.npatch) break;
+        batch_patch<R>(A, A.order[q], s_red, &s_key);
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        __threadfence();
+        if (atomicAdd(&A.queue[1], 1u) == gridDim.x - 1) {
+            atomicExch(&A.queue[0], 0u);
+            atomicExch(&A.queue[1], 0u);
+        }
+    }
+}
+
+void launch_batch(const BatchArgs& a, int nblocks, hipStream_t s) {
+    hipLaunchKernelGGL((k_batch<RT>), dim3(nblocks), dim3(BATCH_THREADS), 0, s, a);
+}
+int batch_blocks_per_cu() {
+    int n = 0;
+    hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_batch<RT>, BATCH_THREADS, 0);
+    return e == hipSuccess && n > 0 ? n : 1;
+}
+
+void launch_step(const StepCtx& c, hipStream_t s) {
+    if (c.dim == 1) c.mhd ? step_t<1, true, RT>(c, s) : step_t<1, false, RT>(c, s);
+    else if (c.dim == 2) c.mhd ? step_t<2, true, RT>(c, s) : step_t<2, false, RT>(c, s);
+    else c.mhd ? step_t<3, true, RT>(c, s) : step_t<3, false, RT>(c, s);
+}
+
+}  // namespace DG_NS
 }  // namespace dg

@@ -206,15 +206,19 @@ def rotor_params(npatch=4096, seed=20260705):
     return n, B0, r0, u0
 
 
-def rotor_2d(n, B0=5.0, r0=0.1, u0=2.0, gamma=1.4, limiter=0):
+def rotor_2d(n, B0=5.0, r0=0.1, u0=2.0, gamma=1.4, limiter=0, ny=None):
     """MHD rotor on [0,1]^2 with outflow boundaries (Examples/MHDRotor):
     ambient rho=1, p=1, Bx = B0/sqrt(4 pi); disk r<r0 rho=10 spinning at
-    edge speed u0, linear taper to ambient by r1 = 1.15 r0."""
+    edge speed u0, linear taper to ambient by r1 = 1.15 r0.  ny: cells in y
+    (default n), dy = 1/ny."""
+    ny = n if ny is None else ny
     d = 1.0 / n
+    dy = 1.0 / ny
     g = ghosts_for(True)
-    w, B = empty_state((n, n), g, True)
+    w, B = empty_state((n, ny), g, True)
     x = centers(n, d) - 0.5
-    X, Y = np.meshgrid(x, x, indexing="ij")
+    y = centers(ny, dy) - 0.5
+    X, Y = np.meshgrid(x, y, indexing="ij")
     r = np.sqrt(X * X + Y * Y)
     r1 = 1.15 * r0
     om = u0 / r0
@@ -223,15 +227,31 @@ def rotor_2d(n, B0=5.0, r0=0.1, u0=2.0, gamma=1.4, limiter=0):
     taper = (r >= r0) & (r < r1)
     rho = np.where(inside, 10.0, np.where(taper, 1.0 + 9.0 * f, 1.0))
     scale = np.where(inside, om, np.where(taper, om * f * r0 / np.maximum(r, 1e-300), 0.0))
-    wi = interior(w, g, (n, n))
+    wi = interior(w, g, (n, ny))
     wi[..., 0] = rho
     wi[..., 1] = 1.0
     wi[..., 2] = -scale * Y
     wi[..., 3] = scale * X
-    Bi = interior(B, g, (n, n), extra=1)
+    Bi = interior(B, g, (n, ny), extra=1)
     Bi[..., 0] = B0 / math.sqrt(4.0 * math.pi)
     cfg = dict(mhd=1, gamma=gamma, solver=5, limiter=limiter, ctu=1, ct_energy=2)
-    return (n, n), (d, d), cfg, [], w, B, {"init_B": True}
+    return (n, ny), (d, dy), cfg, [], w, B, {"init_B": True}
+
+
+def rotor_batch(npatch=4096, seed=20260705):
+    """The batched rotor workload: per-patch sizes and parameters from
+    rotor_params; returns (sizes, spacings, cfg, make) where make(p) builds
+    patch p's (w, B) as rotor_2d does."""
+    n, B0, r0, u0 = rotor_params(npatch, seed)
+    sizes = np.stack([n, n], axis=1)
+    d = 1.0 / sizes
+    cfg = rotor_2d(8)[2]
+
+    def make(p):
+        pb = rotor_2d(int(n[p]), float(B0[p]), float(r0[p]), float(u0[p]))
+        return pb[4], pb[5]
+
+    return sizes, d, cfg, make
 
 
 # ----------------------------------------- reference accuracy examples

@@ -132,8 +132,55 @@ void* dg_stream(dg_grid* g);
  * enable: 1 start (resets), 0 stop, -1 read only.  step_ms: summed device
  * time from the first to the last kernel of each timed step. */
 int dg_timing(dg_grid* g, int enable, double* step_ms, long* steps);
+/* Per-kernel-class device time accumulated while timing is on (events after
+ * each launch).  id in [0, count): returns the class name ("boundary", "cfl",
+ * "pred_sweep_x", ..., "apply"), total ms and launches; DG_ERR_ARG past the
+ * last class.  Instrumentation of this library, no reference counterpart. */
+int dg_kernel_time(dg_grid* g, int id, const char** name, double* ms, long* launches);
 /* Diagnostics: max over interior cells of |div B| * min(dx) / max|B_face| (0 for hydro / 1D). */
 int dg_divb(dg_grid* g, double* out);
+
+/* ---------------------------------------------------------------------
+ * Batched patches.  Replaces the per-child dispatch of DistGrid<T>::step
+ * (Refinement/DistGrid.cpp:291-303, children of DistGrid.hpp:20, each child
+ * running Grid2D::unsplit_step on a DRAGONWING thread) for many independent
+ * 2D MHD grids (CTU + CT, 3 ghosts, the physics switches of `physics`): one
+ * persistent launch advances every patch by one step of its own CFL dt,
+ * with the dt halving of Grid::advance (CFL.cpp:151-213) done per patch on
+ * the device.  Unlike DistGrid children, patches do not share a dt or halos.
+ * Boundaries: bc_kind DG_BC_OUTFLOW or DG_BC_PERIODIC on every face of every
+ * patch, corner ghosts on.  Patch sides must be >= 4 cells.
+ * --------------------------------------------------------------------- */
+typedef struct dg_batch dg_batch;
+int dg_batch_create(const dg_desc* physics, int npatch, const int* nx, const int* ny, const double* dx, const double* dy,
+                    int bc_kind, dg_batch** out);
+int dg_batch_destroy(dg_batch* b);
+/* Interior zones summed over patches; ghost width (3). */
+long dg_batch_zones(const dg_batch* b);
+int dg_batch_ghosts(const dg_batch* b);
+/* Patch state in the single-grid record layout (dg_upload): cells incl.
+ * ghosts, stride 8, and face B (3 per face).  Upload makes it current. */
+int dg_batch_upload(dg_batch* b, int patch, const double* w, int stride, const double* B);
+int dg_batch_download(dg_batch* b, int patch, double* w, int stride, double* B);
+/* initialize_B_fields (MHD/B_fields.cpp:117-149) on every patch. */
+int dg_batch_init_B(dg_batch* b);
+/* nsteps steps of every patch, asynchronous on the batch stream; use_graph:
+ * the nsteps launches are captured once into a HIP graph and replayed.
+ * device_ms (optional): wait and return the device time of the nsteps. */
+int dg_batch_run(dg_batch* b, int nsteps, int use_graph, double* device_ms);
+/* Wait; DG_OK, or the status of the first patch whose last step could not
+ * complete (st->i = patch index; the patch keeps its previous state). */
+int dg_batch_sync(dg_batch* b, dg_status* st);
+int dg_batch_patch_stats(dg_batch* b, int patch, double* dt_last, long* steps, long* fails, int* status);
+/* Size classes: largest side < 64, < 128, < 192, >= 192.  A tile is the
+ * number of x faces one thread walks in the x sweep of a patch of that class.
+ * autotune times each candidate on the class's patches (steps not committed)
+ * and keeps the fastest; class_ms (optional) gets the best time per launch. */
+int dg_batch_set_tiles(dg_batch* b, const int* chunk, int n);
+int dg_batch_get_tiles(const dg_batch* b, int* chunk, int n);
+int dg_batch_autotune(dg_batch* b, int reps, int* chunk_out, double* class_ms);
+const char* dg_batch_last_error(const dg_batch* b);
+void* dg_batch_stream(dg_batch* b);
 
 #ifdef __cplusplus
 }

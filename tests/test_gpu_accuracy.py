@@ -1,0 +1,71 @@
+"""End-to-end accuracy against the reference's published validation tables,
+run on the GPU's FP64 path (bit-identical to the oracle on the parity
+cases).  The runs follow main.cpp:133-150 frame by frame (advance(frame)
+while time < final_time) with each example's Config.h switches:
+
+- Examples/Diagonal/Outputs/Validation_data.md: L1(rho) of the diagonal
+  sine advection (hydro, HLLC, CTU, periodic) after t = 1;
+- Examples/LoopAdvection/Outputs/Validation_data.md: L1 of |B|^2 error / B0^2
+  of the field loop (MHD, HLLD + CT + CTU) after t = 4.
+
+The tables print six significant digits, so agreement is asked to 1e-5
+relative."""
+import numpy as np
+import pytest
+
+from dragon_amd import problems as P
+
+from helpers import gpu_grid
+
+pytestmark = pytest.mark.gpu
+REL = 1e-5
+
+
+def run_frames(pb):
+    gg = gpu_grid(pb)
+    meta = pb[6]
+    fr, t = P.frames(meta["t_end"], meta["frame"])
+    w0 = gg.get_w()
+    steps = 0
+    for f in fr:
+        n, _ = gg.advance(f)
+        steps += n
+    return gg, w0, t, steps
+
+
+@pytest.mark.parametrize("n,limiter,published", [
+    (128, 1, 0.000224884),   # HLLC + MC
+    (256, 0, 0.000286346),   # HLLC + minmod
+    (128, 3, 0.000693177),   # HLLC + superbee
+])
+def test_diagonal_2d_published_l1(n, limiter, published):
+    pb = P.diagonal_2d(n, limiter)
+    gg, _, t, steps = run_frames(pb)
+    l1, l2, linf = P.diagonal_error(gg.get_w(), gg.g, pb[0], t)
+    print(f"diagonal 2D n={n} limiter={limiter}: {steps} steps, L1 {l1:.6g} (published {published})", flush=True)
+    assert abs(l1 - published) <= REL * published
+
+
+def test_diagonal_3d_published_l1():
+    pb = P.diagonal_3d(64, 1)
+    gg, _, t, steps = run_frames(pb)
+    l1, _, _ = P.diagonal_error(gg.get_w(), gg.g, pb[0], t)
+    print(f"diagonal 3D n=64 MC: {steps} steps, L1 {l1:.6g} (published 0.00116811)", flush=True)
+    assert abs(l1 - 0.00116811) <= REL * 0.00116811
+
+
+@pytest.mark.parametrize("n,published", [(64, 0.0286387), (128, 0.0172391)])
+def test_loop_advection_2d_published_l1(n, published):
+    pb = P.loop_advection(n, 2)
+    gg, w0, t, steps = run_frames(pb)
+    l1, _, _ = P.loop_error(gg.get_w(), w0, gg.g, pb[0], pb[6]["B0"])
+    print(f"loop 2D n={n}: {steps} steps, L1 {l1:.6g} (published {published}), div B {gg.divb():.3g}", flush=True)
+    assert abs(l1 - published) <= REL * published
+
+
+def test_loop_advection_3d_gaussian_published_l1():
+    pb = P.loop_advection(64, 3, gaussian=True)
+    gg, w0, t, steps = run_frames(pb)
+    l1, _, _ = P.loop_error(gg.get_w(), w0, gg.g, pb[0], pb[6]["B0"])
+    print(f"loop 3D gaussian n=64: {steps} steps, L1 {l1:.6g} (published 0.323758)", flush=True)
+    assert abs(l1 - 0.323758) <= REL * 0.323758

@@ -1,0 +1,140 @@
+"""Batched patches (one persistent launch per step, per-patch dt) against
+the oracle: every patch advanced by the batch must be bit-identical to the
+same patch advanced alone by the oracle's Grid::advance, one CFL step at a
+time; graph replays must equal eager launches; a patch that cannot step
+reports its failure without disturbing the others."""
+import numpy as np
+import pytest
+
+import oracle as O
+from dragon_amd import problems as P
+
+from helpers import oracle_cfg
+
+pytestmark = pytest.mark.gpu
+
+# (nx, ny, B0, r0, u0): several size classes, non-square patches, x and y
+# faces beyond one wavefront
+PATCHES = [(16, 16, 4.0, 0.1, 1.5), (24, 20, 3.0, 0.09, 1.2), (33, 17, 5.0, 0.12, 2.0), (40, 40, 2.5, 0.1, 1.0),
+           (12, 30, 4.5, 0.11, 1.8), (70, 66, 3.5, 0.1, 1.4), (20, 131, 4.0, 0.08, 1.6), (130, 24, 2.8, 0.12, 1.1)]
+
+
+def build(precision=0, patches=PATCHES):
+    import dragon_amd as D
+    pbs = [P.rotor_2d(nx, B0, r0, u0, ny=ny) for nx, ny, B0, r0, u0 in patches]
+    cfg = D.Config(precision=precision)
+    for k, v in pbs[0][2].items():
+        setattr(cfg, k, v)
+    b = D.Batch([pb[0] for pb in pbs], [pb[1] for pb in pbs], cfg)
+    for p, pb in enumerate(pbs):
+        b.set_state(p, pb[4], pb[5])
+    b.initialize_B_fields()
+    return b, pbs
+
+
+def oracle_patch(pb):
+    shape, d, cfg, bcs, w, B, meta = pb
+    G = O.Grid(shape, d, oracle_cfg(cfg), ghosts=1)
+    G.set_w(w)
+    G.set_B(B)
+    G.init_B()
+    return G
+
+
+def test_batch_fp64_matches_oracle_per_patch():
+    b, pbs = build(0)
+    assert b.g == 3
+    b.run(3)
+    b.run(2, graph=True)
+    b.sync()
+    for p, pb in enumerate(pbs):
+        G = oracle_patch(pb)
+        for _ in range(5):
+            G.advance(1e9, True, 1)
+        w, B = b.get_state(p)
+        st = b.stats(p)
+        same_w = np.array_equal(w, G.get_w())
+        same_B = np.array_equal(B, G.get_B())
+        print(f"patch {p} {pb[0]}: steps {st['steps']} fails {st['fails']} dt {st['dt']:.4g} cells equal {same_w} faces equal {same_B}",
+              flush=True)
+        assert st["steps"] == 5 and st["status"] == 0
+        assert same_w and same_B
+
+
+def test_batch_graph_replays_match_eager():
+    a, _ = build(0)
+    g, _ = build(0)
+    for rep in range(3):
+        a.run(2)
+        a.sync()
+        g.run(2, graph=True)
+        g.sync()
+        for p in range(len(PATCHES)):
+            wa, Ba = a.get_state(p)
+            wg, Bg = g.get_state(p)
+            assert np.array_equal(wa, wg) and np.array_equal(Ba, Bg), (rep, p)
+    print("graph replays == eager launches over 3 replays", flush=True)
+
+
+def test_batch_tiles_do_not_change_results():
+    a, _ = build(0)
+    t = a.tiles()
+    a.run(2)
+    a.sync()
+    b, _ = build(0)
+    tiles, ms = b.autotune(reps=1)
+    print("autotuned tiles", tiles, "ms per class launch", ms, flush=True)
+    assert all(1 <= x <= 4096 for x in tiles)
+    b.set_tiles([3, 5, 7, 9])
+    b.run(2)
+    b.sync()
+    for p in range(len(PATCHES)):
+        wa, Ba = a.get_state(p)
+        wb, Bb = b.get_state(p)
+        assert np.array_equal(wa, wb) and np.array_equal(Ba, Bb), p
+
+
+def test_batch_mixed_close_to_fp64():
+    a, pbs = build(0)
+    m, _ = build(1)
+    a.run(10)
+    m.run(10)
+    a.sync()
+    m.sync()
+    for p, pb in enumerate(pbs):
+        g = 3
+        wa, Ba = a.get_state(p)
+        wm, Bm = m.get_state(p)
+        sl = tuple(slice(g, g + s) for s in pb[0])
+        ra, rm = wa[sl][..., 0], wm[sl][..., 0]
+        l1 = np.abs(rm - ra).sum() / np.abs(ra).sum()
+        Bi = Bm[tuple(slice(g, g + s + 1) for s in pb[0])]
+        d = pb[1]
+        div = (Bi[1:, :-1, 0] - Bi[:-1, :-1, 0]) / d[0] + (Bi[:-1, 1:, 1] - Bi[:-1, :-1, 1]) / d[1]
+        ndiv = np.abs(div).max() * min(d) / np.abs(Bi[..., :2]).max()
+        print(f"patch {p}: mixed rel L1(rho) {l1:.3g}, normalised max|div B| {ndiv:.3g}", flush=True)
+        assert l1 <= 1e-5 and ndiv <= 1e-12
+
+
+def test_batch_failed_patch_reports_and_keeps_state():
+    """A patch with a negative pressure cannot step at any dt: its dt halves
+    on the device down to the minimum, it reports DG_ERR_DT_UNDERFLOW and
+    keeps its state; the other patches step normally."""
+    import dragon_amd as D
+    b, pbs = build(0, PATCHES[:4])
+    w, B = b.get_state(2)
+    w[3 + 5, 3 + 6, 1] = -1.0
+    b.set_state(2, w, B)
+    before = b.get_state(2)
+    b.run(1)
+    with pytest.raises(D.DragonError) as e:
+        b.sync()
+    assert e.value.code == D.DG_ERR_DT_UNDERFLOW and e.value.where == (2,)
+    after = b.get_state(2)
+    assert np.array_equal(before[0], after[0]) and np.array_equal(before[1], after[1])
+    st = b.stats(2)
+    assert st["status"] == D.DG_ERR_DT_UNDERFLOW and st["steps"] == 0 and st["fails"] >= 30
+    for p in (0, 1, 3):
+        G = oracle_patch(pbs[p])
+        G.advance(1e9, True, 1)
+        assert np.array_equal(b.get_state(p)[0], G.get_w()), p
