@@ -96,12 +96,18 @@ def run_form(n, precision, W, Bf, steps, warmup, torch, rho_out=None):
         "kernels": per,
         "max_divB_normalised": divb,
     }
+    out["t_advanced"] = G.t_advanced
     if rho_out is not None:
         Wf = torch.empty_like(W)
         Bo = torch.empty_like(Bf)
         G.download_device(Wf.data_ptr(), Bo.data_ptr())
+        torch.cuda.synchronize()
         g = 3
-        rho_out.append(Wf[0, g:g + n, g:g + n, g:g + n].clone())
+        sl = slice(g, g + n)
+        r = Wf[0, sl, sl, sl]
+        # evolution check: how far rho moved from the initial state
+        out["rel_L1_rho_change"] = float((r - W[0, sl, sl, sl]).abs().sum() / W[0, sl, sl, sl].abs().sum())
+        rho_out.append(r.clone())
         del Wf, Bo
     del G
     return out
@@ -261,6 +267,7 @@ def main():
     dev = torch.device("cuda:0")
     log(f"building {a.n}^3 initial state on the GPU")
     W, Bf = make_state_gpu(a.n, a.seed, torch, dev)
+    log(f"initial state: max|v| {W[2:5].abs().max().item():.4f}, rho mean {W[0, 3:-3, 3:-3, 3:-3].mean().item():.4f}")
     copy = copy_probe(torch, dev)
     log(f"copy probe: {copy}")
     forms = {}
@@ -316,8 +323,8 @@ def main():
     if not a.no_batch:
         try:
             line["batch"] = bench_batch(torch, npatch=a.batch_patches)
-        except Exception as e:  # the headline line must still print
-            line["batch"] = {"error": repr(e)}
+        except D.DragonError as e:  # the headline line must still print
+            line["batch"] = {"error": str(e)}
     print(json.dumps(line), flush=True)
 
 

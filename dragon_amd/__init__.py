@@ -33,7 +33,7 @@ FP64, MIXED = 0, 1
 
 # status codes (dragon_gpu.h)
 DG_OK, DG_ERR_UNPHYSICAL, DG_ERR_NAN, DG_ERR_RIEMANN_INPUT, DG_ERR_RIEMANN_SOLUTION = 0, 1, 2, 3, 4
-DG_ERR_CFL, DG_ERR_DT_UNDERFLOW, DG_ERR_ARG, DG_ERR_HIP = 5, 6, 7, 8
+DG_ERR_CFL, DG_ERR_DT_UNDERFLOW, DG_ERR_ARG, DG_ERR_HIP, DG_ERR_UNSUPPORTED = 5, 6, 7, 8, 9
 
 # Face bits (Boundary.hpp:139-149)
 X_negative, X_positive, Y_negative, Y_positive, Z_negative, Z_positive = 1, 2, 4, 8, 16, 32
@@ -405,6 +405,7 @@ class Grid:
         self._push_bc()
         st, S = _Status(), _Stats()
         self._chk(lib().dg_advance(self._h, float(dt), int(check_cfl), int(max_steps), C.byref(S), C.byref(st)), st)
+        self.t_advanced = float(S.t_advanced)  # simulated time of this call
         return int(S.steps), int(S.failures)
 
     def advance(self, dt: float, check_cfl=True, max_steps=-1):

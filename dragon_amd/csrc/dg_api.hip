@@ -33,6 +33,9 @@ struct dg_grid {
     double *HS = nullptr, *F = nullptr, *Fp = nullptr, *T = nullptr;
     bool fused = true;
     double *E0 = nullptr, *Eh = nullptr;
+    double* cflpart = nullptr;  // per-block CFL maxima of the committed state, written by the apply pass
+    long ncflpart = 0;
+    bool cfl_cached = false;    // cflpart describes the current W (set by a committed step)
     unsigned long long* dev_words = nullptr;   // [0] failure key, [1] CFL max bits, [2..3] diagnostics
     unsigned long long* host_words = nullptr;  // pinned mirror
     std::string err;
@@ -139,6 +142,7 @@ static StepCtx make_ctx(dg_grid* g, double dt) {
     if (g->T)
         for (int q = 0; q < 5; q++) c.T[q] = soa(g->T + (long)q * g->nf * nc, nc);
     c.err = g->dev_words;
+    c.cflpart = g->cflpart;
     c.mark = g->timing ? tm_mark : nullptr;
     c.mark_user = g;
     return c;
@@ -184,6 +188,10 @@ int dg_create(const dg_desc* desc, dg_grid** out) {
     if (!d.mhd && d.dim == 3 && d.ctu) ok = ok && alloc(&g->T, 5L * g->nf * nc);
     if (faces) ok = ok && alloc(&g->B, 3 * nfc) && alloc(&g->Bn, 3 * nfc) && alloc(&g->Bh, 3 * nfc) && alloc(&g->E0, 3 * nfc) &&
                     alloc(&g->Eh, 3 * nfc);
+    if (d.dim >= 2) {
+        g->ncflpart = ((long)nc + 255) / 256;
+        ok = ok && alloc(&g->cflpart, g->ncflpart);
+    }
     ok = ok && hipMalloc((void**)&g->dev_words, 4 * sizeof(unsigned long long)) == hipSuccess;
     ok = ok && hipHostMalloc((void**)&g->host_words, 4 * sizeof(unsigned long long), hipHostMallocDefault) == hipSuccess;
     if (!ok) {
@@ -209,7 +217,7 @@ int dg_destroy(dg_grid* g) {
     if (!g) return DG_OK;
     int rc = DG_OK;
     if (g->s && hipStreamSynchronize(g->s) != hipSuccess) rc = DG_ERR_HIP;
-    double* bufs[] = {g->W, g->Wn, g->B, g->Bn, g->Bh, g->HS, g->F, g->Fp, g->T, g->E0, g->Eh};
+    double* bufs[] = {g->W, g->Wn, g->B, g->Bn, g->Bh, g->HS, g->F, g->Fp, g->T, g->E0, g->Eh, g->cflpart};
     for (double* p : bufs)
         if (p && hipFree(p) != hipSuccess) rc = DG_ERR_HIP;
     if (g->dev_words && hipFree(g->dev_words) != hipSuccess) rc = DG_ERR_HIP;
@@ -231,6 +239,7 @@ void* dg_stream(dg_grid* g) { return g ? (void*)g->s : nullptr; }
 
 int dg_upload(dg_grid* g, const double* w, int stride, const double* B) {
     if (!g || !w || stride < g->nf) return DG_ERR_ARG;
+    g->cfl_cached = false;
     const long nc = g->ce.size;
     double* stage = nullptr;
     HIPCHK(hipMalloc((void**)&stage, sizeof(double) * nc * stride));
@@ -273,6 +282,7 @@ int dg_download(dg_grid* g, double* w, int stride, double* B) {
 
 int dg_upload_device(dg_grid* g, const double* w_soa, const double* B_soa) {
     if (!g || !w_soa) return DG_ERR_ARG;
+    g->cfl_cached = false;
     HIPCHK(hipMemcpyAsync(g->W, w_soa, sizeof(double) * 8 * g->ce.size, hipMemcpyDeviceToDevice, g->s));
     if (B_soa && g->B) HIPCHK(hipMemcpyAsync(g->B, B_soa, sizeof(double) * 3 * g->fe.size, hipMemcpyDeviceToDevice, g->s));
     HIPCHK(hipStreamSynchronize(g->s));
@@ -358,7 +368,8 @@ int dg_cfl_dt(dg_grid* g, double* dt, dg_status* st) {
     launch_fill_u64(g->dev_words + 1, 0ull, 1, g->s);
     StepCtx c = make_ctx(g, 0.0);
     tm_record(g, -1);
-    launch_cfl(c, g->dev_words + 1, g->s);
+    if (g->cfl_cached) launch_cfl_tail(c, g->ncflpart, g->dev_words + 1, g->s);
+    else launch_cfl(c, g->dev_words + 1, g->s);
     tm_record(g, K_CFL);
     HIPCHK(hipMemcpyAsync(g->host_words + 1, g->dev_words + 1, sizeof(unsigned long long), hipMemcpyDeviceToHost, g->s));
     HIPCHK(hipStreamSynchronize(g->s));
@@ -406,6 +417,7 @@ static void decode(const dg_grid* g, unsigned long long key, dg_status* st, int*
 
 int dg_step(dg_grid* g, double dt, dg_status* st) {
     if (!g) return DG_ERR_ARG;
+    g->cfl_cached = false;  // the apply pass overwrites the maxima
     launch_fill_u64(g->dev_words, NO_ERR, 1, g->s);
     StepCtx c = make_ctx(g, dt);
     if (g->timing) HIPCHK(hipEventRecord(g->ev0, g->s));
@@ -432,6 +444,7 @@ int dg_step(dg_grid* g, double dt, dg_status* st) {
     }
     std::swap(g->W, g->Wn);
     if (g->B) std::swap(g->B, g->Bn);
+    g->cfl_cached = g->cflpart != nullptr;
     set_status(st, DG_OK, 0, 0, 0, "");
     return DG_OK;
 }
@@ -475,6 +488,7 @@ int dg_advance(dg_grid* g, double dt, int check_cfl, long max_steps, dg_stats* s
 
 int dg_init_B(dg_grid* g) {
     if (!g) return DG_ERR_ARG;
+    g->cfl_cached = false;
     int rc = dg_apply_boundary(g);
     if (rc) return rc;
     if (!(g->d.mhd && g->d.dim >= 2)) return DG_OK;
@@ -599,6 +613,16 @@ static void batch_launch(dg_batch* b, const int* order, int n, int commit, int i
     const BatchArgs a = batch_args(b, order, n, commit, init_only);
     if (b->d.precision == DG_MIXED) p32::launch_batch(a, blocks, b->s);
     else p64::launch_batch(a, blocks, b->s);
+}
+
+// The persistent batch kernel (k_batch) faulted on MI355X (memory aperture
+// violation) and a CPU emulation of it under AddressSanitizer found no
+// out-of-bounds access, so its cause is not identified: its launches are
+// switched off until it is.
+static const bool kPersistentBatchEnabled = false;
+static int batch_disabled(dg_batch* b) {
+    b->err_msg = "the persistent batched step is switched off: its kernel faulted on the GPU and the cause is not identified";
+    return DG_ERR_UNSUPPORTED;
 }
 
 static void batch_drop_graph(dg_batch* b) {
@@ -736,6 +760,7 @@ int dg_batch_download(dg_batch* b, int patch, double* w, int stride, double* B) 
 
 int dg_batch_init_B(dg_batch* b) {
     if (!b) return DG_ERR_ARG;
+    if (!kPersistentBatchEnabled) return batch_disabled(b);
     batch_launch(b, b->d_order, b->npatch, 1, 1);
     BCHK(hipGetLastError());
     BCHK(hipStreamSynchronize(b->s));
@@ -744,6 +769,7 @@ int dg_batch_init_B(dg_batch* b) {
 
 int dg_batch_run(dg_batch* b, int nsteps, int use_graph, double* device_ms) {
     if (!b || nsteps < 0) return DG_ERR_ARG;
+    if (!kPersistentBatchEnabled) return batch_disabled(b);
     if (nsteps == 0) return DG_OK;
     if (use_graph && (!b->gexec || b->graph_steps != nsteps)) {
         batch_drop_graph(b);
@@ -816,6 +842,7 @@ int dg_batch_get_tiles(const dg_batch* b, int* chunk, int n) {
 
 int dg_batch_autotune(dg_batch* b, int reps, int* chunk_out, double* class_ms) {
     if (!b || reps < 1) return DG_ERR_ARG;
+    if (!kPersistentBatchEnabled) return batch_disabled(b);
     static const int cand[] = {2, 4, 8, 16, 32, 64, 128};
     for (int c = 0; c < BATCH_CLASSES; c++) {
         const int n = (int)b->cls_order[c].size();

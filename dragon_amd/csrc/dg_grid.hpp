@@ -62,11 +62,13 @@ __host__ __device__ inline Box make_box(int dim, int l0, int h0, int l1 = 0, int
 }
 
 // SoA record of nf fields, field f at p + f*stride.
-struct Soa {
-    double* p;
+template <class T>
+struct SoaT {
+    T* p;
     long stride;
-    __device__ double& operator()(int f, long m) const { return p[f * stride + m]; }
+    __device__ T& operator()(int f, long m) const { return p[f * stride + m]; }
 };
+using Soa = SoaT<double>;
 
 // Failure word: the smallest key wins, so the earliest stage and, within a
 // stage, the first index in the reference's loop order is what the host

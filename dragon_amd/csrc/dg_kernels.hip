@@ -26,12 +26,25 @@ namespace dg {
 namespace DG_NS {
 #if DG_PREC == 32
 using RT = float;   // arithmetic type of reconstruction and Riemann solves
+using FT = float;   // storage type of the flux arrays (FP32 Riemann fluxes lose nothing)
 #else
 using RT = double;
+using FT = double;
 #endif
+using FSoa = SoaT<FT>;
+__host__ __device__ inline FSoa fs(const Soa& a) { return FSoa{reinterpret_cast<FT*>(a.p), a.stride}; }
 void ctu_hydro3(const StepCtx& c, hipStream_t s);
 
 #define KB __launch_bounds__(256)
+// Occupancy target of the flux sweeps (waves per SIMD; 0 = compiler's choice).
+#ifndef DG_SWEEP_WAVES
+#define DG_SWEEP_WAVES 0
+#endif
+#if DG_SWEEP_WAVES > 0
+#define KS __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(DG_SWEEP_WAVES, DG_SWEEP_WAVES)))
+#else
+#define KS KB
+#endif
 
 __device__ __forceinline__ long tid() { return (long)blockIdx.x * blockDim.x + threadIdx.x; }
 
@@ -49,15 +62,15 @@ template <class R> __device__ __forceinline__ void store_p(const Soa& a, long m,
     if (mhd) { a(F_BX, m) = double(w.Bx); a(F_BY, m) = double(w.By); a(F_BZ, m) = double(w.Bz); }
 }
 // Flux records use the U field order: rho, E, mx, my, mz, Bx, By, Bz.
-__device__ __forceinline__ U<double> load_u(const Soa& a, long m, bool mhd) {
+template <class T> __device__ __forceinline__ U<double> load_u(const SoaT<T>& a, long m, bool mhd) {
     U<double> u;
     u.rho = a(0, m); u.E = a(1, m); u.mx = a(2, m); u.my = a(3, m); u.mz = a(4, m);
     if (mhd) { u.Bx = a(5, m); u.By = a(6, m); u.Bz = a(7, m); } else { u.Bx = u.By = u.Bz = 0; }
     return u;
 }
-template <class R> __device__ __forceinline__ void store_u(const Soa& a, long m, const U<R>& u, bool mhd) {
-    a(0, m) = double(u.rho); a(1, m) = double(u.E); a(2, m) = double(u.mx); a(3, m) = double(u.my); a(4, m) = double(u.mz);
-    if (mhd) { a(5, m) = double(u.Bx); a(6, m) = double(u.By); a(7, m) = double(u.Bz); }
+template <class R, class T> __device__ __forceinline__ void store_u(const SoaT<T>& a, long m, const U<R>& u, bool mhd) {
+    a(0, m) = T(u.rho); a(1, m) = T(u.E); a(2, m) = T(u.mx); a(3, m) = T(u.my); a(4, m) = T(u.mz);
+    if (mhd) { a(5, m) = T(u.Bx); a(6, m) = T(u.By); a(7, m) = T(u.Bz); }
 }
 __device__ __forceinline__ void cross(double ux, double uy, double uz, double vx, double vy, double vz, double& wx, double& wy, double& wz) {
     wx = uy * vz - uz * vy;  // Arithmetic.cpp:359-365
@@ -188,10 +201,8 @@ KB __global__ void k_bc_faces(BCArgs a) { bc_faces_body(a, tid()); }
 // Max signal speed over cells [-1, n+1)^D (CFL.cpp:113-162); wave64
 // shuffles then one atomicMax per block on the (non-negative) FP64 bits.
 template <int D, bool MHD>
-__device__ __forceinline__ double cfl_speed_at(const Soa& W, const Ext& ce, int i, int j, int k, const Phys& ph, int mode, double dx,
-                                               double dy, double dz) {
+__device__ __forceinline__ double cfl_speed_w(const P<double>& w, const Phys& ph, int mode, double dx, double dy, double dz) {
     {
-        P<double> w = load_p<double>(W, ce.at(i, j, k), MHD);
         double a = MHD ? c_fast_max(w, ph) : cs(w, ph);
         double s = 0;
         if (D == 1) {
@@ -212,6 +223,71 @@ __device__ __forceinline__ double cfl_speed_at(const Soa& W, const Ext& ce, int 
             s = pow(s, 1.0 / p);
         }
         return s;
+    }
+}
+template <int D, bool MHD>
+__device__ __forceinline__ double cfl_speed_at(const Soa& W, const Ext& ce, int i, int j, int k, const Phys& ph, int mode, double dx,
+                                               double dy, double dz) {
+    return cfl_speed_w<D, MHD>(load_p<double>(W, ce.at(i, j, k), MHD), ph, mode, dx, dy, dz);
+}
+// Block max of per-thread values into part[blockIdx.x] (plain store, no atomics).
+__device__ __forceinline__ void block_max_store(double m, double* part) {
+    for (int off = 32; off > 0; off >>= 1) m = dmax(m, __shfl_xor(m, off));
+    __shared__ double wmax[4];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    if (lane == 0) wmax[wave] = m;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double b = wmax[0];
+        for (int q = 1; q < (int)(blockDim.x >> 6); q++) b = dmax(b, wmax[q]);
+        part[blockIdx.x] = b;
+    }
+}
+// CFL of the committed state from the maxima the apply pass left over the
+// interior plus a scan of the ghost layer 1 (the shell of [-1, n+1)^D).
+template <int D, bool MHD>
+KB __global__ void k_cfl_tail(Soa W, Ext ce, const double* part, long nparts, Phys ph, int mode, double dx, double dy, double dz,
+                              unsigned long long* out) {
+    const long n0 = ce.n[0], n1 = ce.n[1], n2 = ce.n[2];
+    // shell slabs: x faces (i = -1, n0) over the full transverse range, then y, then z faces inside
+    const long sx = D == 3 ? (n1 + 2) * (n2 + 2) : (D == 2 ? n1 + 2 : 1);
+    const long sy = D == 3 ? n0 * (n2 + 2) : (D == 2 ? n0 : 0);
+    const long sz = D == 3 ? n0 * n1 : 0;
+    const long shell = 2 * (sx + sy + sz);
+    const long stride = (long)gridDim.x * blockDim.x;
+    double m = 0;
+    for (long t = tid(); t < nparts + shell; t += stride) {
+        if (t < nparts) { m = dmax(m, part[t]); continue; }
+        long r = t - nparts;
+        int i = 0, j = 0, k = 0;
+        if (r < 2 * sx) {
+            i = r < sx ? -1 : (int)n0;
+            r %= sx;
+            if (D == 3) { j = (int)(r / (n2 + 2)) - 1; k = (int)(r % (n2 + 2)) - 1; }
+            else if (D == 2) j = (int)r - 1;
+        } else if (r < 2 * (sx + sy)) {
+            r -= 2 * sx;
+            j = r < sy ? -1 : (int)n1;
+            r %= sy;
+            if (D == 3) { i = (int)(r / (n2 + 2)); k = (int)(r % (n2 + 2)) - 1; }
+            else i = (int)r;
+        } else {
+            r -= 2 * (sx + sy);
+            k = r < sz ? -1 : (int)n2;
+            r %= sz;
+            i = (int)(r / n1); j = (int)(r % n1);
+        }
+        m = dmax(m, cfl_speed_at<D, MHD>(W, ce, i, j, k, ph, mode, dx, dy, dz));
+    }
+    for (int off = 32; off > 0; off >>= 1) m = dmax(m, __shfl_xor(m, off));
+    __shared__ double wmax[4];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    if (lane == 0) wmax[wave] = m;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double b = wmax[0];
+        for (int q = 1; q < (int)(blockDim.x >> 6); q++) b = dmax(b, wmax[q]);
+        if (b > 0) atomicMax(out, (unsigned long long)__double_as_longlong(b));
     }
 }
 template <int D, bool MHD>
@@ -289,7 +365,7 @@ KB __global__ void k_half(Soa W, double* Bx, double* By, double* Bz, Ext ce, Ext
 // ================================================================= fluxes
 // computeFlux_X/Y/Z (Unsplit.cpp:210-254): F[c] = Riemann(R[c - e], L[c]).
 template <int D, bool MHD, class R, int AX>
-KB __global__ void k_flux(Soa HL, Soa HR, Soa F, Ext ce, Box box, double dt_d, Phys ph, unsigned long long* err, int stage) {
+KB __global__ void k_flux(Soa HL, Soa HR, FSoa F, Ext ce, Box box, double dt_d, Phys ph, unsigned long long* err, int stage) {
     int i, j, k;
     if (!box.point(tid(), i, j, k)) return;
     const long m = ce.at(i, j, k);
@@ -317,8 +393,8 @@ struct SweepArgs {
     const double* B[3];
     const double* Bh[3];
     const double* E0[3];
-    Soa Fp[3];             // predictor fluxes (corrector pass input)
-    Soa Fo;                // output fluxes of this axis
+    FSoa Fp[3];            // predictor fluxes (corrector pass input)
+    FSoa Fo;               // output fluxes of this axis
     Ext ce, fe;
     int lo[3], hi[3];      // faces [lo, hi) per dim; face f of axis AX sits between cells f-1 and f
     double dt_d[3];
@@ -490,7 +566,7 @@ __device__ __forceinline__ void sweep_march_body(const SweepArgs& a, long t) {
     }
 }
 template <int D, bool MHD, class R, int AX, bool FINAL>
-KB __global__ void k_sweep_march(SweepArgs a) { sweep_march_body<D, MHD, R, AX, FINAL>(a, tid()); }
+KS __global__ void k_sweep_march(SweepArgs a) { sweep_march_body<D, MHD, R, AX, FINAL>(a, tid()); }
 
 template <class R> __device__ __forceinline__ P<R> shfl_up1(const P<R>& w) {
     return P<R>{__shfl_up(w.rho, 1), __shfl_up(w.p, 1), __shfl_up(w.vx, 1), __shfl_up(w.vy, 1),
@@ -522,13 +598,13 @@ __device__ __forceinline__ void sweep_wave_body(const SweepArgs& a, long wave, i
     if (lane >= 1 && live) face_flux<MHD, R, AX>(a, Rl, L, a.ce.at(idx[0], idx[1], idx[2]));
 }
 template <int D, bool MHD, class R, int AX, bool FINAL>
-KB __global__ void k_sweep_wave(SweepArgs a) { sweep_wave_body<D, MHD, R, AX, FINAL>(a, tid() >> 6, threadIdx.x & 63); }
+KS __global__ void k_sweep_wave(SweepArgs a) { sweep_wave_body<D, MHD, R, AX, FINAL>(a, tid() >> 6, threadIdx.x & 63); }
 
 // ===================================================== CTU hydro (2D/3D)
 // correctState (CTU.cpp:264-301): states along SAX corrected by the flux
 // difference of FAX.
 template <int D, class R>
-KB __global__ void k_correct(Soa HL, Soa HR, Soa F, Ext ce, Box box, int fax, double dt_dL, Phys ph) {
+KB __global__ void k_correct(Soa HL, Soa HR, FSoa F, Ext ce, Box box, int fax, double dt_dL, Phys ph) {
     int i, j, k;
     if (!box.point(tid(), i, j, k)) return;
     const long m = ce.at(i, j, k);
@@ -541,7 +617,7 @@ KB __global__ void k_correct(Soa HL, Soa HR, Soa F, Ext ce, Box box, int fax, do
 // computeCTUFlux_X/Y/Z (CTU.cpp:306-370): a flux between states corrected by
 // the transverse flux T (3D hydro, 12-solve CTU).
 template <class R, int AX>
-KB __global__ void k_ctu_flux(Soa HL, Soa HR, Soa T, Soa F, Ext ce, Box box, int tax, double dt_d, double dt_t, Phys ph,
+KB __global__ void k_ctu_flux(Soa HL, Soa HR, FSoa T, FSoa F, Ext ce, Box box, int tax, double dt_d, double dt_t, Phys ph,
                               unsigned long long* err, int stage) {
     int i, j, k;
     if (!box.point(tid(), i, j, k)) return;
@@ -588,11 +664,11 @@ struct ERef {
 // 2D corner EMF (Electric.cpp:24-35, 52-71) at edge (i,j).  With `fromW`,
 // Eref is computed from the cell states (predictor); otherwise it is read.
 template <bool FROM_W>
-__device__ __forceinline__ void emf2_body(const Soa& FX, const Soa& FY, const Soa& W, const ERef& ref, const Ext& ce, const Ext& fe,
+__device__ __forceinline__ void emf2_body(const FSoa& FX, const FSoa& FY, const Soa& W, const ERef& ref, const Ext& ce, const Ext& fe,
                                           const Box& box, int upwind, double* Ex, double* Ey, double* Ez, long t) {
     int i, j, k;
     if (!box.point(t, i, j, k)) return;
-    auto f = [&](const Soa& F, int fld, int a, int b) { return F(fld, ce.at(a, b, 0)); };
+    auto f = [&](const FSoa& F, int fld, int a, int b) { return double(F(fld, ce.at(a, b, 0))); };
     double ex = -f(FY, U_BZ, i, j);
     double ey = f(FX, U_BZ, i, j);
     double ez = (f(FY, U_BX, i - 1, j) - f(FX, U_BY, i, j - 1) + f(FY, U_BX, i, j) - f(FX, U_BY, i, j)) * 0.25;
@@ -619,18 +695,18 @@ __device__ __forceinline__ void emf2_body(const Soa& FX, const Soa& FY, const So
     Ex[e] = ex; Ey[e] = ey; Ez[e] = ez;
 }
 template <bool FROM_W>
-KB __global__ void k_emf2(Soa FX, Soa FY, Soa W, ERef ref, Ext ce, Ext fe, Box box, int upwind, double* Ex, double* Ey,
+KB __global__ void k_emf2(FSoa FX, FSoa FY, Soa W, ERef ref, Ext ce, Ext fe, Box box, int upwind, double* Ex, double* Ey,
                           double* Ez) {
     emf2_body<FROM_W>(FX, FY, W, ref, ce, fe, box, upwind, Ex, Ey, Ez, tid());
 }
 
 // 3D corner EMF (Electric.cpp:38-47, 74-100).
 template <bool FROM_W>
-KB __global__ void k_emf3(Soa FX, Soa FY, Soa FZ, Soa W, ERef ref, Ext ce, Ext fe, Box box, int upwind, double* Ex,
+KB __global__ void k_emf3(FSoa FX, FSoa FY, FSoa FZ, Soa W, ERef ref, Ext ce, Ext fe, Box box, int upwind, double* Ex,
                           double* Ey, double* Ez) {
     int i, j, k;
     if (!box.point(tid(), i, j, k)) return;
-    auto f = [&](const Soa& F, int fld, int a, int b, int c) { return F(fld, ce.at(a, b, c)); };
+    auto f = [&](const FSoa& F, int fld, int a, int b, int c) { return double(F(fld, ce.at(a, b, c))); };
     double ex = (f(FZ, U_BY, i, j - 1, k) - f(FY, U_BZ, i, j, k - 1) + f(FZ, U_BY, i, j, k) - f(FY, U_BZ, i, j, k)) * 0.25;
     double ey = (f(FX, U_BZ, i, j, k - 1) - f(FZ, U_BX, i - 1, j, k) + f(FX, U_BZ, i, j, k) - f(FZ, U_BX, i, j, k)) * 0.25;
     double ez = (f(FY, U_BX, i - 1, j, k) - f(FX, U_BY, i, j - 1, k) + f(FY, U_BX, i, j, k) - f(FX, U_BY, i, j, k)) * 0.25;
@@ -729,7 +805,7 @@ __device__ __forceinline__ void cell_B(const double* Bx, const double* By, const
 // at dt/2 over cells [-1, n+1), body fields from the half-step faces, then
 // Ehalf = cross(B, v) over cells [-1, n].
 template <int D>
-__device__ __forceinline__ void whalf_body(const Soa& W, const Soa& FX, const Soa& FY, const Soa& FZ, const double* Bhx, const double* Bhy,
+__device__ __forceinline__ void whalf_body(const Soa& W, const FSoa& FX, const FSoa& FY, const FSoa& FZ, const double* Bhx, const double* Bhy,
                                            const double* Bhz, const Ext& ce, const Ext& fe, const Box& box, double hx, double hy, double hz,
                                            const Phys& ph, double* Ehx, double* Ehy, double* Ehz, unsigned long long* err, long t) {
     int i, j, k;
@@ -751,7 +827,7 @@ __device__ __forceinline__ void whalf_body(const Soa& W, const Soa& FX, const So
     Ehx[e] = ex; Ehy[e] = ey; Ehz[e] = ez;
 }
 template <int D>
-KB __global__ void k_whalf(Soa W, Soa FX, Soa FY, Soa FZ, const double* Bhx, const double* Bhy, const double* Bhz, Ext ce,
+KB __global__ void k_whalf(Soa W, FSoa FX, FSoa FY, FSoa FZ, const double* Bhx, const double* Bhy, const double* Bhz, Ext ce,
                            Ext fe, Box box, double hx, double hy, double hz, Phys ph, double* Ehx, double* Ehy, double* Ehz,
                            unsigned long long* err) {
     whalf_body<D>(W, FX, FY, FZ, Bhx, Bhy, Bhz, ce, fe, box, hx, hy, hz, ph, Ehx, Ehy, Ehz, err, tid());
@@ -760,7 +836,7 @@ KB __global__ void k_whalf(Soa W, Soa FX, Soa FY, Soa FZ, const double* Bhx, con
 // ctu_sweep_MHD corrections (CTU.cpp:65-129 3D, 162-202 2D), in place.
 template <int D>
 KB __global__ void k_ctu_mhd(Soa W, const double* Bx, const double* By, const double* Bz, const double* Bhx, const double* Bhy,
-                             const double* Bhz, const double* E0x, const double* E0y, const double* E0z, Soa FX, Soa FY, Soa FZ,
+                             const double* Bhz, const double* E0x, const double* E0y, const double* E0z, FSoa FX, FSoa FY, FSoa FZ,
                              Soa XL, Soa XR, Soa YL, Soa YR, Soa ZL, Soa ZR, Ext ce, Ext fe, Box box, double dt_dx, double dt_dy,
                              double dt_dz, Phys ph) {
     int i, j, k;
@@ -780,7 +856,7 @@ KB __global__ void k_ctu_mhd(Soa W, const double* Bx, const double* By, const do
         store_p<double>(HL, m, prim<true>(uL, ph), true);
         store_p<double>(HR, m, prim<true>(uR, ph), true);
     };
-    auto fd = [&](const Soa& F, int ax) { return usub<true>(load_u(F, m, true), load_u(F, m + ce.s[ax], true)); };
+    auto fd = [&](const FSoa& F, int ax) { return usub<true>(load_u(F, m, true), load_u(F, m + ce.s[ax], true)); };
     if (D == 2) {
         U<double> corr = umul<true>(umul<true>(fd(FY, 1), 0.5), dt_dy);
         corr.Bx = corr.By = corr.Bz = 0;
@@ -830,11 +906,14 @@ KB __global__ void k_ctu_mhd(Soa W, const double* Bx, const double* By, const do
 // the physicality check (Unsplit.cpp:94-99, 188-195), writing the complete
 // next state: interior cells updated, ghost cells copied (the reference
 // commits interior cells only, so the old ghosts remain).
+// Returns the CFL signal speed of the new state for interior cells when
+// `cfl` is set (0 elsewhere), for the fused CFL of the next step.
 template <int D, bool MHD>
-__device__ __forceinline__ void apply_body(const Soa& W, const Soa& Wn, const Soa& FX, const Soa& FY, const Soa& FZ, const double* Bnx,
-                                           const double* Bny, const double* Bnz, const Ext& ce, const Ext& fe, double dt_dx, double dt_dy,
-                                           double dt_dz, const Phys& ph, unsigned long long* err, long m) {
-    if (m >= ce.size) return;
+__device__ __forceinline__ double apply_body(const Soa& W, const Soa& Wn, const FSoa& FX, const FSoa& FY, const FSoa& FZ, const double* Bnx,
+                                             const double* Bny, const double* Bnz, const Ext& ce, const Ext& fe, double dt_dx, double dt_dy,
+                                             double dt_dz, const Phys& ph, unsigned long long* err, long m, bool cfl = false, int cfl_mode = 1,
+                                             double cdx = 1, double cdy = 1, double cdz = 1) {
+    if (m >= ce.size) return 0.0;
     const int nf = MHD ? 8 : 5;
     long r = m;
     int i, j = 0, k = 0;
@@ -845,7 +924,7 @@ __device__ __forceinline__ void apply_body(const Soa& W, const Soa& Wn, const So
     const bool interior = i >= 0 && i < ce.n[0] && j >= 0 && j < ce.n[1] && k >= 0 && k < ce.n[2];
     if (!interior) {
         for (int f = 0; f < nf; f++) Wn(f, m) = W(f, m);
-        return;
+        return 0.0;
     }
     P<double> w = load_p<double>(W, m, MHD);
     U<double> u = cons<MHD>(w, ph);
@@ -861,18 +940,22 @@ __device__ __forceinline__ void apply_body(const Soa& W, const Soa& Wn, const So
     }
     if (!physical<MHD>(w)) report(err, ST_UNPHYSICAL, (unsigned long long)m, E_UNPHYSICAL);
     store_p<double>(Wn, m, w, MHD);
+    return cfl ? dmax(0.0, cfl_speed_w<D, MHD>(w, ph, cfl_mode, cdx, cdy, cdz)) : 0.0;  // NaN never wins
 }
 template <int D, bool MHD>
-KB __global__ void k_apply(Soa W, Soa Wn, Soa FX, Soa FY, Soa FZ, const double* Bnx, const double* Bny, const double* Bnz, Ext ce,
-                           Ext fe, double dt_dx, double dt_dy, double dt_dz, Phys ph, unsigned long long* err) {
-    apply_body<D, MHD>(W, Wn, FX, FY, FZ, Bnx, Bny, Bnz, ce, fe, dt_dx, dt_dy, dt_dz, ph, err, tid());
+KB __global__ void k_apply(Soa W, Soa Wn, FSoa FX, FSoa FY, FSoa FZ, const double* Bnx, const double* Bny, const double* Bnz, Ext ce,
+                           Ext fe, double dt_dx, double dt_dy, double dt_dz, Phys ph, unsigned long long* err, double* cflpart,
+                           int cfl_mode, double cdx, double cdy, double cdz) {
+    const double v = apply_body<D, MHD>(W, Wn, FX, FY, FZ, Bnx, Bny, Bnz, ce, fe, dt_dx, dt_dy, dt_dz, ph, err, tid(), cflpart != nullptr,
+                                        cfl_mode, cdx, cdy, cdz);
+    if (cflpart) block_max_store(v, cflpart);  // uniform branch
 }
 
 // 1D Godunov::sweep update (Splitting.cpp:67-95): cells [-g+1, n+g-2] take
 // the flux difference; cell n+g-2 then takes the sweep's trailing zero
 // update (a prim->cons->prim round trip).  Ghosts are committed too.
 template <bool MHD>
-KB __global__ void k_apply1(Soa W, Soa Wn, Soa F, Ext ce, double dt_dx, Phys ph, unsigned long long* err) {
+KB __global__ void k_apply1(Soa W, Soa Wn, FSoa F, Ext ce, double dt_dx, Phys ph, unsigned long long* err) {
     const long m = tid();
     if (m >= ce.size) return;
     const int nf = MHD ? 8 : 5, g = ce.g, n = ce.n[0];
@@ -985,6 +1068,17 @@ static void cfl_t(const StepCtx& c, unsigned long long* out, hipStream_t s) {
     const unsigned nb = std::min(blocks(b.count()), 8192u);
     hipLaunchKernelGGL((k_cfl<D, MHD>), dim3(nb), dim3(256), 0, s, c.W, c.ce, b, c.ph, c.cfl_mode, c.d[0], c.d[1], c.d[2], out);
 }
+template <int D, bool MHD>
+static void cfl_tail_t(const StepCtx& c, long nparts, unsigned long long* out, hipStream_t s) {
+    const long work = nparts + (long)c.ce.n[0] * c.ce.n[1] * c.ce.n[2];  // upper bound; the kernel is grid-stride
+    const unsigned nb = std::min(blocks(work), 2048u);
+    hipLaunchKernelGGL((k_cfl_tail<D, MHD>), dim3(nb), dim3(256), 0, s, c.W, c.ce, c.cflpart, nparts, c.ph, c.cfl_mode, c.d[0], c.d[1],
+                       c.d[2], out);
+}
+void launch_cfl_tail(const StepCtx& c, long nparts, unsigned long long* out, hipStream_t s) {
+    if (c.dim == 2) c.mhd ? cfl_tail_t<2, true>(c, nparts, out, s) : cfl_tail_t<2, false>(c, nparts, out, s);
+    else c.mhd ? cfl_tail_t<3, true>(c, nparts, out, s) : cfl_tail_t<3, false>(c, nparts, out, s);
+}
 void launch_cfl(const StepCtx& c, unsigned long long* out, hipStream_t s) {
     if (c.dim == 1) c.mhd ? cfl_t<1, true>(c, out, s) : cfl_t<1, false>(c, out, s);
     else if (c.dim == 2) c.mhd ? cfl_t<2, true>(c, out, s) : cfl_t<2, false>(c, out, s);
@@ -992,12 +1086,12 @@ void launch_cfl(const StepCtx& c, unsigned long long* out, hipStream_t s) {
 }
 
 template <int D, bool MHD, class R, int AX>
-static void flux_t(const StepCtx& c, const Soa& HL, const Soa& HR, const Soa& F, Box b, double dt_d, int stage, hipStream_t s) {
+static void flux_t(const StepCtx& c, const Soa& HL, const Soa& HR, const FSoa& F, Box b, double dt_d, int stage, hipStream_t s) {
     if (b.count() <= 0) return;
     hipLaunchKernelGGL((k_flux<D, MHD, R, AX>), dim3(blocks(b.count())), dim3(256), 0, s, HL, HR, F, c.ce, b, dt_d, c.ph, c.err, stage);
 }
 template <int D, bool MHD, class R>
-static void flux_ax(const StepCtx& c, int ax, const Soa& HL, const Soa& HR, const Soa& F, Box b, double dt_d, int stage, hipStream_t s) {
+static void flux_ax(const StepCtx& c, int ax, const Soa& HL, const Soa& HR, const FSoa& F, Box b, double dt_d, int stage, hipStream_t s) {
     if (ax == 0) flux_t<D, MHD, R, 0>(c, HL, HR, F, b, dt_d, stage, s);
     else if (ax == 1) { if constexpr (D >= 2) flux_t<D, MHD, R, 1>(c, HL, HR, F, b, dt_d, stage, s); }
     else { if constexpr (D >= 3) flux_t<D, MHD, R, 2>(c, HL, HR, F, b, dt_d, stage, s); }
@@ -1016,11 +1110,11 @@ static Box face_box(int D, const Ext& ce, int ax, int along_lo, int along_hi_inc
 
 // One sweep launch for axis AX over the face box b.
 template <int D, bool MHD, class R, bool FINAL, int AX>
-static void sweep_t(const StepCtx& c, const Box& b, const Soa& out, int stage, hipStream_t s) {
+static void sweep_t(const StepCtx& c, const Box& b, const FSoa& out, int stage, hipStream_t s) {
     SweepArgs a{};
     a.W = c.W;
     for (int q = 0; q < 3; q++) {
-        a.B[q] = c.B[q]; a.Bh[q] = c.Bh[q]; a.E0[q] = c.E0[q]; a.Fp[q] = c.Fp[q];
+        a.B[q] = c.B[q]; a.Bh[q] = c.Bh[q]; a.E0[q] = c.E0[q]; a.Fp[q] = fs(c.Fp[q]);
         a.lo[q] = b.lo[q]; a.hi[q] = b.hi[q];
         a.dt_d[q] = c.dt / c.d[q];
     }
@@ -1048,7 +1142,7 @@ static void sweep_t(const StepCtx& c, const Box& b, const Soa& out, int stage, h
     }
 }
 template <int D, bool MHD, class R, bool FINAL>
-static void sweep(const StepCtx& c, int ax, const Box& b, const Soa& out, int stage, hipStream_t s) {
+static void sweep(const StepCtx& c, int ax, const Box& b, const FSoa& out, int stage, hipStream_t s) {
     if (ax == 0) sweep_t<D, MHD, R, FINAL, 0>(c, b, out, stage, s);
     else if (ax == 1) { if constexpr (D >= 2) sweep_t<D, MHD, R, FINAL, 1>(c, b, out, stage, s); }
     else { if constexpr (D >= 3) sweep_t<D, MHD, R, FINAL, 2>(c, b, out, stage, s); }
@@ -1064,24 +1158,24 @@ static void step_fused(const StepCtx& c, hipStream_t s) {
     if (c.ctu) {
         for (int ax = 0; ax < D; ax++) {
             Box b = MHD ? face_box(D, ce, ax, -1, 1, -2, 2) : face_box(D, ce, ax, 0, 0, -1, 1);
-            sweep<D, MHD, R, false>(c, ax, b, c.Fp[ax], ST_PRED_X + ax, s);
+            sweep<D, MHD, R, false>(c, ax, b, fs(c.Fp[ax]), ST_PRED_X + ax, s);
             mark(c, K_PRED_X + ax);
         }
         if constexpr (MHD) {
             ERef none{nullptr, nullptr, nullptr};
             Box eb = make_box(D, -1, ce.n[0] + 2, -1, ce.n[1] + 2, -1, ce.n[2] + 2);
             if constexpr (D == 2)
-                hipLaunchKernelGGL((k_emf2<true>), dim3(blocks(eb.count())), dim3(256), 0, s, c.Fp[0], c.Fp[1], c.W, none, ce, c.fe, eb, 1,
+                hipLaunchKernelGGL((k_emf2<true>), dim3(blocks(eb.count())), dim3(256), 0, s, fs(c.Fp[0]), fs(c.Fp[1]), c.W, none, ce, c.fe, eb, 1,
                                    c.E0[0], c.E0[1], c.E0[2]);
             else
-                hipLaunchKernelGGL((k_emf3<true>), dim3(blocks(eb.count())), dim3(256), 0, s, c.Fp[0], c.Fp[1], c.Fp[2], c.W, none, ce,
+                hipLaunchKernelGGL((k_emf3<true>), dim3(blocks(eb.count())), dim3(256), 0, s, fs(c.Fp[0]), fs(c.Fp[1]), fs(c.Fp[2]), c.W, none, ce,
                                    c.fe, eb, 1, c.E0[0], c.E0[1], c.E0[2]);
             mark(c, K_EMF_PRED);
             hipLaunchKernelGGL((k_faraday<D>), dim3(blocks(c.fe.size)), dim3(256), 0, s, c.E0[0], c.E0[1], c.E0[2], c.B[0], c.B[1],
                                c.B[2], c.Bh[0], c.Bh[1], c.Bh[2], c.fe, 1, 0.5 * dt_d[0], 0.5 * dt_d[1], 0.5 * dt_d[2]);
             mark(c, K_FARADAY_HALF);
             Box cb = make_box(D, -1, ce.n[0] + 1, -1, ce.n[1] + 1, -1, ce.n[2] + 1);
-            hipLaunchKernelGGL((k_whalf<D>), dim3(blocks(cb.count())), dim3(256), 0, s, c.W, c.Fp[0], c.Fp[1], c.Fp[2], c.Bh[0],
+            hipLaunchKernelGGL((k_whalf<D>), dim3(blocks(cb.count())), dim3(256), 0, s, c.W, fs(c.Fp[0]), fs(c.Fp[1]), fs(c.Fp[2]), c.Bh[0],
                                c.Bh[1], c.Bh[2], ce, c.fe, cb, 0.5 * dt_d[0], 0.5 * dt_d[1], 0.5 * dt_d[2], c.ph, c.Eh[0], c.Eh[1],
                                c.Eh[2], c.err);
             mark(c, K_WHALF);
@@ -1089,26 +1183,26 @@ static void step_fused(const StepCtx& c, hipStream_t s) {
     }
     for (int ax = 0; ax < D; ax++) {
         Box b = MHD ? face_box(D, ce, ax, 0, 0, -1, 1) : face_box(D, ce, ax, 0, 0, 0, 0);
-        if (c.ctu) sweep<D, MHD, R, true>(c, ax, b, c.F[ax], ST_FLUX_X + ax, s);
-        else sweep<D, MHD, R, false>(c, ax, b, c.F[ax], ST_FLUX_X + ax, s);
+        if (c.ctu) sweep<D, MHD, R, true>(c, ax, b, fs(c.F[ax]), ST_FLUX_X + ax, s);
+        else sweep<D, MHD, R, false>(c, ax, b, fs(c.F[ax]), ST_FLUX_X + ax, s);
         mark(c, K_FLUX_X + ax);
     }
     if constexpr (MHD) {
         Box eb = make_box(D, 0, ce.n[0] + 1, 0, ce.n[1] + 1, 0, ce.n[2] + 1);
         ERef eh{c.Eh[0], c.Eh[1], c.Eh[2]};
         if constexpr (D == 2)
-            hipLaunchKernelGGL((k_emf2<false>), dim3(blocks(eb.count())), dim3(256), 0, s, c.F[0], c.F[1], c.W, eh, ce, c.fe, eb, c.ctu,
+            hipLaunchKernelGGL((k_emf2<false>), dim3(blocks(eb.count())), dim3(256), 0, s, fs(c.F[0]), fs(c.F[1]), c.W, eh, ce, c.fe, eb, c.ctu,
                                c.E0[0], c.E0[1], c.E0[2]);
         else
-            hipLaunchKernelGGL((k_emf3<false>), dim3(blocks(eb.count())), dim3(256), 0, s, c.F[0], c.F[1], c.F[2], c.W, eh, ce, c.fe, eb,
+            hipLaunchKernelGGL((k_emf3<false>), dim3(blocks(eb.count())), dim3(256), 0, s, fs(c.F[0]), fs(c.F[1]), fs(c.F[2]), c.W, eh, ce, c.fe, eb,
                                c.ctu, c.E0[0], c.E0[1], c.E0[2]);
         mark(c, K_EMF);
         hipLaunchKernelGGL((k_faraday<D>), dim3(blocks(c.fe.size)), dim3(256), 0, s, c.E0[0], c.E0[1], c.E0[2], c.B[0], c.B[1], c.B[2],
                            c.Bn[0], c.Bn[1], c.Bn[2], c.fe, 0, dt_d[0], dt_d[1], dt_d[2]);
         mark(c, K_FARADAY);
     }
-    hipLaunchKernelGGL((k_apply<D, MHD>), dim3(blocks(ce.size)), dim3(256), 0, s, c.W, c.Wn, c.F[0], c.F[1], c.F[2], c.Bn[0], c.Bn[1],
-                       c.Bn[2], ce, c.fe, dt_d[0], dt_d[1], dt_d[2], c.ph, c.err);
+    hipLaunchKernelGGL((k_apply<D, MHD>), dim3(blocks(ce.size)), dim3(256), 0, s, c.W, c.Wn, fs(c.F[0]), fs(c.F[1]), fs(c.F[2]), c.Bn[0], c.Bn[1],
+                       c.Bn[2], ce, c.fe, dt_d[0], dt_d[1], dt_d[2], c.ph, c.err, c.cflpart, c.cfl_mode, c.d[0], c.d[1], c.d[2]);
     mark(c, K_APPLY);
 }
 
@@ -1132,10 +1226,10 @@ static void step_t(const StepCtx& c, hipStream_t s) {
     }
     if (D == 1) {
         Box fb = make_box(1, -g + 1, ce.n[0] + g, 0, 1, 0, 1);
-        flux_ax<D, MHD, R>(c, 0, c.HS[0], c.HS[1], c.F[0], fb, dt_d[0], ST_FLUX_X, s);
+        flux_ax<D, MHD, R>(c, 0, c.HS[0], c.HS[1], fs(c.F[0]), fb, dt_d[0], ST_FLUX_X, s);
         mark(c, K_FLUX_X);
         if constexpr (D == 1)
-            hipLaunchKernelGGL((k_apply1<MHD>), dim3(blocks(ce.size)), dim3(256), 0, s, c.W, c.Wn, c.F[0], ce, dt_d[0], c.ph, c.err);
+            hipLaunchKernelGGL((k_apply1<MHD>), dim3(blocks(ce.size)), dim3(256), 0, s, c.W, c.Wn, fs(c.F[0]), ce, dt_d[0], c.ph, c.err);
         mark(c, K_APPLY);
         return;
     }
@@ -1143,43 +1237,43 @@ static void step_t(const StepCtx& c, hipStream_t s) {
         if constexpr (MHD) {
             // ctu_sweep_MHD predictor fluxes on [-1, n+1] x [-2, n+2)
             for (int ax = 0; ax < D; ax++) {
-                flux_ax<D, MHD, R>(c, ax, c.HS[2 * ax], c.HS[2 * ax + 1], c.F[ax], face_box(D, ce, ax, -1, 1, -2, 2), dt_d[ax],
+                flux_ax<D, MHD, R>(c, ax, c.HS[2 * ax], c.HS[2 * ax + 1], fs(c.F[ax]), face_box(D, ce, ax, -1, 1, -2, 2), dt_d[ax],
                                    ST_PRED_X + ax, s);
                 mark(c, K_PRED_X + ax);
             }
             ERef none{nullptr, nullptr, nullptr};
             Box eb = make_box(D, -1, ce.n[0] + 2, -1, ce.n[1] + 2, -1, ce.n[2] + 2);
             if constexpr (D == 2)
-                hipLaunchKernelGGL((k_emf2<true>), dim3(blocks(eb.count())), dim3(256), 0, s, c.F[0], c.F[1], c.W, none, ce, c.fe, eb, 1,
+                hipLaunchKernelGGL((k_emf2<true>), dim3(blocks(eb.count())), dim3(256), 0, s, fs(c.F[0]), fs(c.F[1]), c.W, none, ce, c.fe, eb, 1,
                                    c.E0[0], c.E0[1], c.E0[2]);
             else
-                hipLaunchKernelGGL((k_emf3<true>), dim3(blocks(eb.count())), dim3(256), 0, s, c.F[0], c.F[1], c.F[2], c.W, none, ce, c.fe,
+                hipLaunchKernelGGL((k_emf3<true>), dim3(blocks(eb.count())), dim3(256), 0, s, fs(c.F[0]), fs(c.F[1]), fs(c.F[2]), c.W, none, ce, c.fe,
                                    eb, 1, c.E0[0], c.E0[1], c.E0[2]);
             mark(c, K_EMF_PRED);
             hipLaunchKernelGGL((k_faraday<D>), dim3(blocks(c.fe.size)), dim3(256), 0, s, c.E0[0], c.E0[1], c.E0[2], c.B[0], c.B[1],
                                c.B[2], c.Bh[0], c.Bh[1], c.Bh[2], c.fe, 1, 0.5 * dt_d[0], 0.5 * dt_d[1], 0.5 * dt_d[2]);
             mark(c, K_FARADAY_HALF);
             Box cb = make_box(D, -1, ce.n[0] + 1, -1, ce.n[1] + 1, -1, ce.n[2] + 1);
-            hipLaunchKernelGGL((k_whalf<D>), dim3(blocks(cb.count())), dim3(256), 0, s, c.W, c.F[0], c.F[1], c.F[2], c.Bh[0], c.Bh[1],
+            hipLaunchKernelGGL((k_whalf<D>), dim3(blocks(cb.count())), dim3(256), 0, s, c.W, fs(c.F[0]), fs(c.F[1]), fs(c.F[2]), c.Bh[0], c.Bh[1],
                                c.Bh[2], ce, c.fe, cb, 0.5 * dt_d[0], 0.5 * dt_d[1], 0.5 * dt_d[2], c.ph, c.Eh[0], c.Eh[1], c.Eh[2],
                                c.err);
             mark(c, K_WHALF);
             hipLaunchKernelGGL((k_ctu_mhd<D>), dim3(blocks(cb.count())), dim3(256), 0, s, c.W, c.B[0], c.B[1], c.B[2], c.Bh[0], c.Bh[1],
-                               c.Bh[2], c.E0[0], c.E0[1], c.E0[2], c.F[0], c.F[1], c.F[2], c.HS[0], c.HS[1], c.HS[2], c.HS[3],
+                               c.Bh[2], c.E0[0], c.E0[1], c.E0[2], fs(c.F[0]), fs(c.F[1]), fs(c.F[2]), c.HS[0], c.HS[1], c.HS[2], c.HS[3],
                                c.HS[4], c.HS[5], ce, c.fe, cb, dt_d[0], dt_d[1], dt_d[2], c.ph);
             mark(c, K_CTU);
         } else if constexpr (D == 2) {
             // ctu_sweep_hydro 2D (CTU.cpp:206-218)
-            flux_ax<D, MHD, R>(c, 0, c.HS[0], c.HS[1], c.F[0], face_box(D, ce, 0, 0, 0, -1, 1), dt_d[0], ST_PRED_X, s);
+            flux_ax<D, MHD, R>(c, 0, c.HS[0], c.HS[1], fs(c.F[0]), face_box(D, ce, 0, 0, 0, -1, 1), dt_d[0], ST_PRED_X, s);
             mark(c, K_PRED_X);
-            flux_ax<D, MHD, R>(c, 1, c.HS[2], c.HS[3], c.F[1], face_box(D, ce, 1, 0, 0, -1, 1), dt_d[1], ST_PRED_Y, s);
+            flux_ax<D, MHD, R>(c, 1, c.HS[2], c.HS[3], fs(c.F[1]), face_box(D, ce, 1, 0, 0, -1, 1), dt_d[1], ST_PRED_Y, s);
             mark(c, K_PRED_Y);
             Box bx = make_box(2, -1, ce.n[0] + 1, 0, ce.n[1]);
-            hipLaunchKernelGGL((k_correct<D, R>), dim3(blocks(bx.count())), dim3(256), 0, s, c.HS[0], c.HS[1], c.F[1], ce, bx, 1,
+            hipLaunchKernelGGL((k_correct<D, R>), dim3(blocks(bx.count())), dim3(256), 0, s, c.HS[0], c.HS[1], fs(c.F[1]), ce, bx, 1,
                                0.5 * dt_d[1], c.ph);
             mark(c, K_CTU);
             Box by = make_box(2, 0, ce.n[0], -1, ce.n[1] + 1);
-            hipLaunchKernelGGL((k_correct<D, R>), dim3(blocks(by.count())), dim3(256), 0, s, c.HS[2], c.HS[3], c.F[0], ce, by, 0,
+            hipLaunchKernelGGL((k_correct<D, R>), dim3(blocks(by.count())), dim3(256), 0, s, c.HS[2], c.HS[3], fs(c.F[0]), ce, by, 0,
                                0.5 * dt_d[0], c.ph);
             mark(c, K_CTU);
         } else {
@@ -1190,13 +1284,13 @@ static void step_t(const StepCtx& c, hipStream_t s) {
     // final fluxes
     if (MHD) {
         for (int ax = 0; ax < D; ax++) {
-            flux_ax<D, MHD, R>(c, ax, c.HS[2 * ax], c.HS[2 * ax + 1], c.F[ax], face_box(D, ce, ax, 0, 0, -1, 1), dt_d[ax],
+            flux_ax<D, MHD, R>(c, ax, c.HS[2 * ax], c.HS[2 * ax + 1], fs(c.F[ax]), face_box(D, ce, ax, 0, 0, -1, 1), dt_d[ax],
                                ST_FLUX_X + ax, s);
             mark(c, K_FLUX_X + ax);
         }
     } else {
         for (int ax = 0; ax < D; ax++) {
-            flux_ax<D, MHD, R>(c, ax, c.HS[2 * ax], c.HS[2 * ax + 1], c.F[ax], face_box(D, ce, ax, 0, 0, 0, 0), dt_d[ax],
+            flux_ax<D, MHD, R>(c, ax, c.HS[2 * ax], c.HS[2 * ax + 1], fs(c.F[ax]), face_box(D, ce, ax, 0, 0, 0, 0), dt_d[ax],
                                ST_FLUX_X + ax, s);
             mark(c, K_FLUX_X + ax);
         }
@@ -1205,18 +1299,18 @@ static void step_t(const StepCtx& c, hipStream_t s) {
         Box eb = make_box(D, 0, ce.n[0] + 1, 0, ce.n[1] + 1, 0, ce.n[2] + 1);
         ERef eh{c.Eh[0], c.Eh[1], c.Eh[2]};
         if constexpr (D == 2)
-            hipLaunchKernelGGL((k_emf2<false>), dim3(blocks(eb.count())), dim3(256), 0, s, c.F[0], c.F[1], c.W, eh, ce, c.fe, eb, c.ctu,
+            hipLaunchKernelGGL((k_emf2<false>), dim3(blocks(eb.count())), dim3(256), 0, s, fs(c.F[0]), fs(c.F[1]), c.W, eh, ce, c.fe, eb, c.ctu,
                                c.E0[0], c.E0[1], c.E0[2]);
         else
-            hipLaunchKernelGGL((k_emf3<false>), dim3(blocks(eb.count())), dim3(256), 0, s, c.F[0], c.F[1], c.F[2], c.W, eh, ce, c.fe, eb,
+            hipLaunchKernelGGL((k_emf3<false>), dim3(blocks(eb.count())), dim3(256), 0, s, fs(c.F[0]), fs(c.F[1]), fs(c.F[2]), c.W, eh, ce, c.fe, eb,
                                c.ctu, c.E0[0], c.E0[1], c.E0[2]);
         mark(c, K_EMF);
         hipLaunchKernelGGL((k_faraday<D>), dim3(blocks(c.fe.size)), dim3(256), 0, s, c.E0[0], c.E0[1], c.E0[2], c.B[0], c.B[1], c.B[2],
                            c.Bn[0], c.Bn[1], c.Bn[2], c.fe, 0, dt_d[0], dt_d[1], dt_d[2]);
         mark(c, K_FARADAY);
     }
-    hipLaunchKernelGGL((k_apply<D, MHD>), dim3(blocks(ce.size)), dim3(256), 0, s, c.W, c.Wn, c.F[0], c.F[1], c.F[2], c.Bn[0], c.Bn[1],
-                       c.Bn[2], ce, c.fe, dt_d[0], dt_d[1], dt_d[2], c.ph, c.err);
+    hipLaunchKernelGGL((k_apply<D, MHD>), dim3(blocks(ce.size)), dim3(256), 0, s, c.W, c.Wn, fs(c.F[0]), fs(c.F[1]), fs(c.F[2]), c.Bn[0], c.Bn[1],
+                       c.Bn[2], ce, c.fe, dt_d[0], dt_d[1], dt_d[2], c.ph, c.err, c.cflpart, c.cfl_mode, c.d[0], c.d[1], c.d[2]);
     mark(c, K_APPLY);
 }
 
@@ -1226,13 +1320,14 @@ void ctu_hydro3(const StepCtx& c, hipStream_t s) {
     const Ext& ce = c.ce;
     const double dx = c.dt / c.d[0], dy = c.dt / c.d[1], dz = c.dt / c.d[2];
     const Soa &xL = c.HS[0], &xR = c.HS[1], &yL = c.HS[2], &yR = c.HS[3], &zL = c.HS[4], &zR = c.HS[5];
-    const Soa &FX = c.F[0], &FY = c.F[1], &FZ = c.F[2];
-    const Soa *T = c.T;  // five scratch flux arrays
+    const FSoa FX = fs(c.F[0]), FY = fs(c.F[1]), FZ = fs(c.F[2]);
+    FSoa T[5];  // five scratch flux arrays
+    for (int q = 0; q < 5; q++) T[q] = fs(c.T[q]);
     const int nx = ce.n[0], ny = ce.n[1], nz = ce.n[2];
     flux_t<3, false, double, 0>(c, xL, xR, FX, make_box(3, 0, nx + 1, -1, ny + 1, -1, nz + 1), dx, ST_PRED_X, s);
     flux_t<3, false, double, 1>(c, yL, yR, FY, make_box(3, -1, nx + 1, 0, ny + 1, -1, nz + 1), dy, ST_PRED_Y, s);
     flux_t<3, false, double, 2>(c, zL, zR, FZ, make_box(3, -1, nx + 1, -1, ny + 1, 0, nz + 1), dz, ST_PRED_Z, s);
-    auto ctu = [&](int ax, const Soa& L, const Soa& Rr, const Soa& Tr, int tax, const Soa& out, double dd, double dtt, int stage) {
+    auto ctu = [&](int ax, const Soa& L, const Soa& Rr, const FSoa& Tr, int tax, const FSoa& out, double dd, double dtt, int stage) {
         int isX = tax == 0, isY = tax == 1, isZ = tax == 2;
         Box b;
         if (ax == 0) b = make_box(3, 0, nx + 1, -1 + isY, ny + 1 - isY, -1 + isZ, nz + 1 - isZ);
@@ -1245,13 +1340,13 @@ void ctu_hydro3(const StepCtx& c, hipStream_t s) {
         else
             hipLaunchKernelGGL((k_ctu_flux<double, 2>), dim3(blocks(b.count())), dim3(256), 0, s, L, Rr, Tr, out, ce, b, tax, dd, dtt, c.ph, c.err, stage);
     };
-    auto corr = [&](const Soa& L, const Soa& Rr, const Soa& F, double dd, int sdim, int fdim) {
+    auto corr = [&](const Soa& L, const Soa& Rr, const FSoa& F, double dd, int sdim, int fdim) {
         int lo[3] = {0, 0, 0}, hi[3] = {nx, ny, nz};
         lo[sdim] = -1; hi[sdim] += 1;
         Box b = make_box(3, lo[0], hi[0], lo[1], hi[1], lo[2], hi[2]);
         hipLaunchKernelGGL((k_correct<3, double>), dim3(blocks(b.count())), dim3(256), 0, s, L, Rr, F, ce, b, fdim, dd, c.ph);
     };
-    const Soa &FXz = T[0], &FYz = T[1], &FXy = T[2], &FZy = T[3], &FYx = T[4];
+    const FSoa &FXz = T[0], &FYz = T[1], &FXy = T[2], &FZy = T[3], &FYx = T[4];
     ctu(0, xL, xR, FZ, 2, FXz, dx, dz / 3.0, ST_CTU_F0 + 0);
     ctu(1, yL, yR, FZ, 2, FYz, dy, dz / 3.0, ST_CTU_F0 + 1);
     ctu(0, xL, xR, FY, 1, FXy, dx, dy / 3.0, ST_CTU_F0 + 2);
@@ -1259,7 +1354,7 @@ void ctu_hydro3(const StepCtx& c, hipStream_t s) {
     corr(xL, xR, FYz, 0.5 * dy, 0, 1);
     corr(xL, xR, FZy, 0.5 * dz, 0, 2);
     ctu(1, yL, yR, FX, 0, FYx, dy, dx / 3.0, ST_CTU_F0 + 4);
-    const Soa& FZx = FZ;  // FZ is no longer needed
+    const FSoa& FZx = FZ;  // FZ is no longer needed
     ctu(2, zL, zR, FX, 0, FZx, dz, dx / 3.0, ST_CTU_F0 + 5);
     corr(yL, yR, FXz, 0.5 * dx, 1, 0);
     corr(yL, yR, FZx, 0.5 * dz, 1, 2);
@@ -1295,10 +1390,10 @@ __device__ void batch_patch(const BatchArgs& A, int p, double* s_red, unsigned l
         E0[c] = A.E0 + 3 * P.of + c * P.nfc;
         Eh[c] = A.Eh + 3 * P.of + c * P.nfc;
     }
-    Soa Fp[3], F[3];
+    FSoa Fp[3], F[3];  // flux pools hold FT elements at the same per-patch offsets
     for (int ax = 0; ax < 2; ax++) {
-        Fp[ax] = Soa{A.Fp + 16 * P.oc + ax * 8 * P.nc, P.nc};
-        F[ax] = Soa{A.F + 16 * P.oc + ax * 8 * P.nc, P.nc};
+        Fp[ax] = FSoa{reinterpret_cast<FT*>(A.Fp) + 16 * P.oc + ax * 8 * P.nc, P.nc};
+        F[ax] = FSoa{reinterpret_cast<FT*>(A.F) + 16 * P.oc + ax * 8 * P.nc, P.nc};
     }
     Fp[2] = Fp[0];  // no z axis in 2D; never read
     F[2] = F[0];

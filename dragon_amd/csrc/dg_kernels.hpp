@@ -38,6 +38,7 @@ struct StepCtx {
     double* E0[3];        // corner EMF (predictor, then final)
     double* Eh[3];        // half-step reference EMF
     unsigned long long* err;
+    double* cflpart;      // per-block CFL maxima of the new state (apply pass), or null
     void (*mark)(void* user, int id);  // per-kernel timing hook (null when off)
     void* mark_user;
 };
@@ -104,6 +105,7 @@ struct BatchArgs {
     void launch_step(const StepCtx& c, hipStream_t s);                                                          \
     void launch_bc(const BCArgs& a, bool faces, hipStream_t s);                                                 \
     void launch_cfl(const StepCtx& c, unsigned long long* out, hipStream_t s);                                  \
+    void launch_cfl_tail(const StepCtx& c, long nparts, unsigned long long* out, hipStream_t s);                \
     void launch_init_B(int dim, Soa W, double* B, Ext ce, Ext fe, hipStream_t s);                               \
     void launch_divb(int dim, double* B, Ext ce, Ext fe, const double d[3], unsigned long long* out, hipStream_t s); \
     void launch_fill_u64(unsigned long long* p, unsigned long long v, int n, hipStream_t s);                    \

@@ -11,7 +11,10 @@ from dragon_amd import problems as P
 
 from helpers import oracle_cfg
 
-pytestmark = pytest.mark.gpu
+pytestmark = [pytest.mark.gpu,
+              pytest.mark.skip(reason="the persistent batched kernel faulted on MI355X (memory aperture violation); a CPU "
+                                      "emulation under AddressSanitizer found no out-of-bounds access, the cause is not "
+                                      "identified, and its launches are switched off")]
 
 # (nx, ny, B0, r0, u0): several size classes, non-square patches, x and y
 # faces beyond one wavefront
