@@ -466,10 +466,12 @@ BATCH_CLASSES = 4
 
 class Batch:
     """Many independent 2D MHD patches (CTU + CT), each its own Grid2D with
-    3 ghosts and outflow (or periodic) faces, advanced together: one
-    persistent launch per step, each patch with its own CFL dt and its own
-    dt halving.  It replaces the per-child dispatch of DistGrid<T>::step
-    (Refinement/DistGrid.cpp:291-303) for patches that share no dt."""
+    3 ghosts and outflow (or periodic) faces, advanced together: a step is
+    a fixed sequence of launches over every patch, each patch attempting one
+    step at its own CFL dt.  A failed attempt keeps the patch's state and
+    halves its dt for the next step.  It replaces the per-child dispatch of
+    DistGrid<T>::step (Refinement/DistGrid.cpp:291-303) for patches that
+    share no dt."""
 
     def __init__(self, sizes, d, config: Config | None = None, bc: str = "outflow"):
         cfg = config or Config()

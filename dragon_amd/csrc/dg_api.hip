@@ -546,31 +546,32 @@ int dg_divb(dg_grid* g, double* out) {
 }  // extern "C"
 
 // ======================================================= batched patches
-// Thousands of independent 2D MHD patches (CTU + CT), all advanced by one
-// persistent launch per step with per-patch dt.  The state of every patch is
-// in pools on the device; patch p is committed by flipping its parity word.
+// Thousands of independent 2D MHD patches (CTU + CT), each with its own CFL
+// dt, advanced together: every stage of the step is one launch over all
+// patches (block tables map workgroups to patches), so one step is a fixed
+// sequence of launches that a HIP graph replays.  Patch p is committed by
+// flipping its parity word.
 struct dg_batch {
     dg_desc d{};
     Phys ph{};
     int npatch = 0, bc_kind = 0;
     std::vector<PatchDesc> P;
-    std::vector<int> order;                         // all patches, largest first
-    std::vector<int> cls_order[BATCH_CLASSES];      // per size class, largest first
+    std::vector<int> order;                         // largest first: block tables list big patches first
     PatchDesc* dP = nullptr;
-    int* d_order = nullptr;                         // [order | class 0 | class 1 | ...]
-    long cls_off[BATCH_CLASSES] = {};
     double *W[2] = {nullptr, nullptr}, *B[2] = {nullptr, nullptr};
     double *Fp = nullptr, *F = nullptr, *Bh = nullptr, *E0 = nullptr, *Eh = nullptr;
     int* parity = nullptr;
     double* dt_last = nullptr;
     long *steps = nullptr, *fails = nullptr;
     int* status = nullptr;
-    unsigned long long* err = nullptr;
-    unsigned* queue = nullptr;
+    unsigned long long *err = nullptr, *vmax = nullptr;
+    double* dtp = nullptr;
+    int *halv = nullptr, *active = nullptr;
     double* stage = nullptr;                        // upload/download staging, largest patch
+    int* bt_mem[BK_COUNT] = {};                     // per kind: [patch of each block | first item of each block]
+    BlockTable bt[BK_COUNT] = {};
     long cells = 0, faces = 0, zones = 0, max_nc = 0, max_nfc = 0;
     int chunk[BATCH_CLASSES] = {8, 16, 32, 32};
-    int resident = 0;                               // workgroups that fit on the device at once
     hipStream_t s = nullptr;
     hipGraph_t graph = nullptr;
     hipGraphExec_t gexec = nullptr;
@@ -593,42 +594,77 @@ static int batch_class(int nx, int ny) {
     return n < 64 ? 0 : n < 128 ? 1 : n < 192 ? 2 : 3;
 }
 
-static BatchArgs batch_args(const dg_batch* b, const int* order, int n, int commit, int init_only) {
-    BatchArgs a{};
-    a.P = b->dP; a.order = order; a.npatch = n;
-    a.W[0] = b->W[0]; a.W[1] = b->W[1]; a.B[0] = b->B[0]; a.B[1] = b->B[1];
-    a.Fp = b->Fp; a.F = b->F; a.Bh = b->Bh; a.E0 = b->E0; a.Eh = b->Eh;
-    a.parity = b->parity; a.dt_last = b->dt_last; a.steps = b->steps; a.fails = b->fails; a.status = b->status;
-    a.err = b->err; a.queue = b->queue;
-    a.ph = b->ph;
-    a.cfl_coeff = b->d.cfl_coeff; a.tol = b->d.timestep_tol > 0 ? b->d.timestep_tol : 1e-14;
-    a.cfl_mode = b->d.cfl_mode; a.bc_kind = b->bc_kind;
-    for (int c = 0; c < BATCH_CLASSES; c++) a.chunk[c] = b->chunk[c];
-    a.commit = commit; a.init_only = init_only;
-    return a;
-}
-
-static void batch_launch(dg_batch* b, const int* order, int n, int commit, int init_only) {
-    const int blocks = std::max(1, std::min(n, b->resident));
-    const BatchArgs a = batch_args(b, order, n, commit, init_only);
-    if (b->d.precision == DG_MIXED) p32::launch_batch(a, blocks, b->s);
-    else p64::launch_batch(a, blocks, b->s);
-}
-
-// The persistent batch kernel (k_batch) faulted on MI355X (memory aperture
-// violation) and a CPU emulation of it under AddressSanitizer found no
-// out-of-bounds access, so its cause is not identified: its launches are
-// switched off until it is.
-static const bool kPersistentBatchEnabled = false;
-static int batch_disabled(dg_batch* b) {
-    b->err_msg = "the persistent batched step is switched off: its kernel faulted on the GPU and the cause is not identified";
-    return DG_ERR_UNSUPPORTED;
+// Work items of one patch for each kind (the ranges of the batched kernels).
+static long batch_items(const dg_batch* b, const PatchDesc& q, int kind) {
+    const long nx = q.n[0], ny = q.n[1], sides = b->bc_kind == DG_BC_PERIODIC ? 1 : 2;
+    auto chunked = [&](long len, long rows) {
+        const long c = std::min<long>(b->chunk[q.cls], len);
+        return ((len + c - 1) / c) * rows;
+    };
+    switch (kind) {
+        case BK_BCC_X: return sides * (ny + 6);
+        case BK_BCC_Y: return sides * (nx + 6);
+        case BK_BCF_X: return sides * (ny + 7);
+        case BK_BCF_Y: return sides * (nx + 7);
+        case BK_CELLS1: return (nx + 2) * (ny + 2);
+        case BK_PRED_X: return chunked(nx + 3, ny + 4);
+        case BK_PRED_Y: return ((ny + 3 + 62) / 63) * (nx + 4) * 64;
+        case BK_EDGES_P: return (nx + 3) * (ny + 3);
+        case BK_FACES: return q.nfc;
+        case BK_FLUX_X: return chunked(nx + 1, ny + 2);
+        case BK_FLUX_Y: return ((ny + 1 + 62) / 63) * (nx + 2) * 64;
+        case BK_EDGES: return (nx + 1) * (ny + 1);
+        default: return q.nc;  // BK_CELLS
+    }
 }
 
 static void batch_drop_graph(dg_batch* b) {
     if (b->gexec) (void)hipGraphExecDestroy(b->gexec);
     if (b->graph) (void)hipGraphDestroy(b->graph);
     b->gexec = nullptr; b->graph = nullptr; b->graph_steps = 0;
+}
+
+// (Re)build the block table of one kind on the device.
+static int batch_table(dg_batch* b, int kind) {
+    std::vector<int> patch, first;
+    for (int p : b->order) {
+        const long items = batch_items(b, b->P[p], kind);
+        for (long f = 0; f < items; f += BATCH_THREADS) {
+            patch.push_back(p);
+            first.push_back((int)f);
+        }
+    }
+    const size_t nb = patch.size();
+    if (b->bt_mem[kind] && hipFree(b->bt_mem[kind]) != hipSuccess) return DG_ERR_HIP;
+    b->bt_mem[kind] = nullptr;
+    if (hipMalloc((void**)&b->bt_mem[kind], sizeof(int) * 2 * std::max<size_t>(nb, 1)) != hipSuccess) return DG_ERR_HIP;
+    if (nb) {
+        if (hipMemcpy(b->bt_mem[kind], patch.data(), sizeof(int) * nb, hipMemcpyHostToDevice) != hipSuccess) return DG_ERR_HIP;
+        if (hipMemcpy(b->bt_mem[kind] + nb, first.data(), sizeof(int) * nb, hipMemcpyHostToDevice) != hipSuccess) return DG_ERR_HIP;
+    }
+    b->bt[kind] = BlockTable{b->bt_mem[kind], b->bt_mem[kind] + nb, (unsigned)nb};
+    return DG_OK;
+}
+
+static BatchArgs batch_args(const dg_batch* b, int commit, int cls_filter) {
+    BatchArgs a{};
+    a.P = b->dP; a.npatch = b->npatch;
+    a.W[0] = b->W[0]; a.W[1] = b->W[1]; a.B[0] = b->B[0]; a.B[1] = b->B[1];
+    a.Fp = b->Fp; a.F = b->F; a.Bh = b->Bh; a.E0 = b->E0; a.Eh = b->Eh;
+    a.parity = b->parity; a.dt_last = b->dt_last; a.steps = b->steps; a.fails = b->fails; a.status = b->status;
+    a.err = b->err; a.vmax = b->vmax; a.dt = b->dtp; a.halv = b->halv; a.active = b->active;
+    a.ph = b->ph;
+    a.cfl_coeff = b->d.cfl_coeff; a.tol = b->d.timestep_tol > 0 ? b->d.timestep_tol : 1e-14;
+    a.cfl_mode = b->d.cfl_mode; a.bc_kind = b->bc_kind;
+    for (int c = 0; c < BATCH_CLASSES; c++) a.chunk[c] = b->chunk[c];
+    a.commit = commit; a.cls_filter = cls_filter;
+    return a;
+}
+
+static void batch_launch(dg_batch* b, int commit, int cls_filter, bool init_only) {
+    const BatchArgs a = batch_args(b, commit, cls_filter);
+    if (b->d.precision == DG_MIXED) p32::launch_batch_step(a, b->bt, init_only, b->s);
+    else p64::launch_batch_step(a, b->bt, init_only, b->s);
 }
 
 extern "C" {
@@ -663,42 +699,33 @@ int dg_batch_create(const dg_desc* physics, int npatch, const int* nx, const int
     b->order.resize(npatch);
     for (int p = 0; p < npatch; p++) b->order[p] = p;
     std::stable_sort(b->order.begin(), b->order.end(), [&](int x, int y) { return b->P[x].nc > b->P[y].nc; });
-    for (int p : b->order) b->cls_order[b->P[p].cls].push_back(p);
-    std::vector<int> all(b->order);
-    for (int c = 0; c < BATCH_CLASSES; c++) {
-        b->cls_off[c] = (long)all.size();
-        all.insert(all.end(), b->cls_order[c].begin(), b->cls_order[c].end());
-    }
     bool ok = hipStreamCreateWithFlags(&b->s, hipStreamNonBlocking) == hipSuccess;
     auto alloc = [&](void** ptr, size_t bytes) { return ok && hipMalloc(ptr, bytes) == hipSuccess; };
-    const size_t cd = sizeof(double) * b->cells, fd = sizeof(double) * b->faces;
+    const size_t cd = sizeof(double) * b->cells, fd = sizeof(double) * b->faces, n = (size_t)npatch;
     ok = alloc((void**)&b->W[0], 8 * cd) && alloc((void**)&b->W[1], 8 * cd) && alloc((void**)&b->B[0], 3 * fd) &&
          alloc((void**)&b->B[1], 3 * fd) && alloc((void**)&b->Fp, 16 * cd) && alloc((void**)&b->F, 16 * cd) &&
          alloc((void**)&b->Bh, 3 * fd) && alloc((void**)&b->E0, 3 * fd) && alloc((void**)&b->Eh, 3 * fd) &&
-         alloc((void**)&b->dP, sizeof(PatchDesc) * npatch) && alloc((void**)&b->d_order, sizeof(int) * all.size()) &&
-         alloc((void**)&b->parity, sizeof(int) * npatch) && alloc((void**)&b->dt_last, sizeof(double) * npatch) &&
-         alloc((void**)&b->steps, sizeof(long) * npatch) && alloc((void**)&b->fails, sizeof(long) * npatch) &&
-         alloc((void**)&b->status, sizeof(int) * npatch) && alloc((void**)&b->err, sizeof(unsigned long long) * npatch) &&
-         alloc((void**)&b->queue, 2 * sizeof(unsigned)) &&
+         alloc((void**)&b->dP, sizeof(PatchDesc) * n) && alloc((void**)&b->parity, sizeof(int) * n) &&
+         alloc((void**)&b->dt_last, sizeof(double) * n) && alloc((void**)&b->steps, sizeof(long) * n) &&
+         alloc((void**)&b->fails, sizeof(long) * n) && alloc((void**)&b->status, sizeof(int) * n) &&
+         alloc((void**)&b->err, sizeof(unsigned long long) * n) && alloc((void**)&b->vmax, sizeof(unsigned long long) * n) &&
+         alloc((void**)&b->dtp, sizeof(double) * n) && alloc((void**)&b->halv, sizeof(int) * n) &&
+         alloc((void**)&b->active, sizeof(int) * n) &&
          alloc((void**)&b->stage, sizeof(double) * std::max(8 * b->max_nc, 3 * b->max_nfc));
-    ok = ok && hipMemcpyAsync(b->dP, b->P.data(), sizeof(PatchDesc) * npatch, hipMemcpyHostToDevice, b->s) == hipSuccess;
-    ok = ok && hipMemcpyAsync(b->d_order, all.data(), sizeof(int) * all.size(), hipMemcpyHostToDevice, b->s) == hipSuccess;
+    ok = ok && hipMemcpyAsync(b->dP, b->P.data(), sizeof(PatchDesc) * n, hipMemcpyHostToDevice, b->s) == hipSuccess;
     // set-up only (never inside a captured graph)
     ok = ok && hipMemsetAsync(b->W[0], 0, 8 * cd, b->s) == hipSuccess && hipMemsetAsync(b->W[1], 0, 8 * cd, b->s) == hipSuccess;
     ok = ok && hipMemsetAsync(b->B[0], 0, 3 * fd, b->s) == hipSuccess && hipMemsetAsync(b->B[1], 0, 3 * fd, b->s) == hipSuccess;
-    ok = ok && hipMemsetAsync(b->parity, 0, sizeof(int) * npatch, b->s) == hipSuccess;
-    ok = ok && hipMemsetAsync(b->dt_last, 0, sizeof(double) * npatch, b->s) == hipSuccess;
-    ok = ok && hipMemsetAsync(b->steps, 0, sizeof(long) * npatch, b->s) == hipSuccess;
-    ok = ok && hipMemsetAsync(b->fails, 0, sizeof(long) * npatch, b->s) == hipSuccess;
-    ok = ok && hipMemsetAsync(b->status, 0, sizeof(int) * npatch, b->s) == hipSuccess;
-    ok = ok && hipMemsetAsync(b->queue, 0, 2 * sizeof(unsigned), b->s) == hipSuccess;
+    ok = ok && hipMemsetAsync(b->parity, 0, sizeof(int) * n, b->s) == hipSuccess;
+    ok = ok && hipMemsetAsync(b->dt_last, 0, sizeof(double) * n, b->s) == hipSuccess;
+    ok = ok && hipMemsetAsync(b->steps, 0, sizeof(long) * n, b->s) == hipSuccess;
+    ok = ok && hipMemsetAsync(b->fails, 0, sizeof(long) * n, b->s) == hipSuccess;
+    ok = ok && hipMemsetAsync(b->status, 0, sizeof(int) * n, b->s) == hipSuccess;
+    ok = ok && hipMemsetAsync(b->halv, 0, sizeof(int) * n, b->s) == hipSuccess;
+    ok = ok && hipMemsetAsync(b->active, 0, sizeof(int) * n, b->s) == hipSuccess;
     ok = ok && hipEventCreate(&b->e0) == hipSuccess && hipEventCreate(&b->e1) == hipSuccess;
     ok = ok && hipStreamSynchronize(b->s) == hipSuccess;
-    if (ok) {
-        int dev = 0, cus = 0;
-        ok = hipGetDevice(&dev) == hipSuccess && hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess;
-        b->resident = std::max(1, cus * (b->d.precision == DG_MIXED ? p32::batch_blocks_per_cu() : p64::batch_blocks_per_cu()));
-    }
+    for (int k = 0; ok && k < BK_COUNT; k++) ok = batch_table(b, k) == DG_OK;
     if (!ok) {
         dg_batch_destroy(b);
         return DG_ERR_HIP;
@@ -712,10 +739,12 @@ int dg_batch_destroy(dg_batch* b) {
     int rc = DG_OK;
     if (b->s && hipStreamSynchronize(b->s) != hipSuccess) rc = DG_ERR_HIP;
     batch_drop_graph(b);
-    void* bufs[] = {b->W[0], b->W[1], b->B[0], b->B[1], b->Fp, b->F, b->Bh, b->E0, b->Eh, b->dP, b->d_order, b->parity,
-                    b->dt_last, b->steps, b->fails, b->status, b->err, b->queue, b->stage};
+    void* bufs[] = {b->W[0], b->W[1], b->B[0], b->B[1], b->Fp, b->F, b->Bh, b->E0, b->Eh, b->dP, b->parity, b->dt_last,
+                    b->steps, b->fails, b->status, b->err, b->vmax, b->dtp, b->halv, b->active, b->stage};
     for (void* p : bufs)
         if (p && hipFree(p) != hipSuccess) rc = DG_ERR_HIP;
+    for (int k = 0; k < BK_COUNT; k++)
+        if (b->bt_mem[k] && hipFree(b->bt_mem[k]) != hipSuccess) rc = DG_ERR_HIP;
     if (b->e0 && hipEventDestroy(b->e0) != hipSuccess) rc = DG_ERR_HIP;
     if (b->e1 && hipEventDestroy(b->e1) != hipSuccess) rc = DG_ERR_HIP;
     if (b->s && hipStreamDestroy(b->s) != hipSuccess) rc = DG_ERR_HIP;
@@ -731,12 +760,20 @@ void* dg_batch_stream(dg_batch* b) { return b ? (void*)b->s : nullptr; }
 int dg_batch_upload(dg_batch* b, int patch, const double* w, int stride, const double* B) {
     if (!b || patch < 0 || patch >= b->npatch || !w || stride != 8 || !B) return DG_ERR_ARG;
     const PatchDesc& q = b->P[patch];
-    const int par = 0;
+    BCHK(hipStreamSynchronize(b->s));
+    int par = 0;
+    BCHK(hipMemcpy(&par, b->parity + patch, sizeof(int), hipMemcpyDeviceToHost));
     BCHK(hipMemcpyAsync(b->stage, w, sizeof(double) * 8 * q.nc, hipMemcpyHostToDevice, b->s));
-    launch_aos_to_soa(b->stage, 8, 8, soa(b->W[par] + 8 * q.oc, q.nc), q.nc, b->s);
+    p64::launch_aos_to_soa(b->stage, 8, 8, soa(b->W[par] + 8 * q.oc, q.nc), q.nc, b->s);
     BCHK(hipMemcpyAsync(b->stage, B, sizeof(double) * 3 * q.nfc, hipMemcpyHostToDevice, b->s));
-    launch_aos_to_soa(b->stage, 3, 3, soa(b->B[par] + 3 * q.of, q.nfc), q.nfc, b->s);
-    BCHK(hipMemcpyAsync(b->parity + patch, &par, sizeof(int), hipMemcpyHostToDevice, b->s));
+    p64::launch_aos_to_soa(b->stage, 3, 3, soa(b->B[par] + 3 * q.of, q.nfc), q.nfc, b->s);
+    // a fresh state: clear the patch's counters and any stopped status
+    const int zero = 0;
+    const long lzero = 0;
+    BCHK(hipMemcpyAsync(b->status + patch, &zero, sizeof(int), hipMemcpyHostToDevice, b->s));
+    BCHK(hipMemcpyAsync(b->halv + patch, &zero, sizeof(int), hipMemcpyHostToDevice, b->s));
+    BCHK(hipMemcpyAsync(b->steps + patch, &lzero, sizeof(long), hipMemcpyHostToDevice, b->s));
+    BCHK(hipMemcpyAsync(b->fails + patch, &lzero, sizeof(long), hipMemcpyHostToDevice, b->s));
     BCHK(hipStreamSynchronize(b->s));
     return DG_OK;
 }
@@ -744,14 +781,14 @@ int dg_batch_upload(dg_batch* b, int patch, const double* w, int stride, const d
 int dg_batch_download(dg_batch* b, int patch, double* w, int stride, double* B) {
     if (!b || patch < 0 || patch >= b->npatch || !w || stride != 8) return DG_ERR_ARG;
     const PatchDesc& q = b->P[patch];
-    int par = 0;
-    BCHK(hipMemcpyAsync(&par, b->parity + patch, sizeof(int), hipMemcpyDeviceToHost, b->s));
     BCHK(hipStreamSynchronize(b->s));
-    launch_soa_to_aos(soa(b->W[par] + 8 * q.oc, q.nc), 8, b->stage, 8, q.nc, b->s);
+    int par = 0;
+    BCHK(hipMemcpy(&par, b->parity + patch, sizeof(int), hipMemcpyDeviceToHost));
+    p64::launch_soa_to_aos(soa(b->W[par] + 8 * q.oc, q.nc), 8, b->stage, 8, q.nc, b->s);
     BCHK(hipMemcpyAsync(w, b->stage, sizeof(double) * 8 * q.nc, hipMemcpyDeviceToHost, b->s));
     BCHK(hipStreamSynchronize(b->s));
     if (B) {
-        launch_soa_to_aos(soa(b->B[par] + 3 * q.of, q.nfc), 3, b->stage, 3, q.nfc, b->s);
+        p64::launch_soa_to_aos(soa(b->B[par] + 3 * q.of, q.nfc), 3, b->stage, 3, q.nfc, b->s);
         BCHK(hipMemcpyAsync(B, b->stage, sizeof(double) * 3 * q.nfc, hipMemcpyDeviceToHost, b->s));
         BCHK(hipStreamSynchronize(b->s));
     }
@@ -760,8 +797,7 @@ int dg_batch_download(dg_batch* b, int patch, double* w, int stride, double* B) 
 
 int dg_batch_init_B(dg_batch* b) {
     if (!b) return DG_ERR_ARG;
-    if (!kPersistentBatchEnabled) return batch_disabled(b);
-    batch_launch(b, b->d_order, b->npatch, 1, 1);
+    batch_launch(b, 1, -1, true);
     BCHK(hipGetLastError());
     BCHK(hipStreamSynchronize(b->s));
     return DG_OK;
@@ -769,12 +805,11 @@ int dg_batch_init_B(dg_batch* b) {
 
 int dg_batch_run(dg_batch* b, int nsteps, int use_graph, double* device_ms) {
     if (!b || nsteps < 0) return DG_ERR_ARG;
-    if (!kPersistentBatchEnabled) return batch_disabled(b);
     if (nsteps == 0) return DG_OK;
     if (use_graph && (!b->gexec || b->graph_steps != nsteps)) {
         batch_drop_graph(b);
         BCHK(hipStreamBeginCapture(b->s, hipStreamCaptureModeThreadLocal));
-        for (int k = 0; k < nsteps; k++) batch_launch(b, b->d_order, b->npatch, 1, 0);
+        for (int k = 0; k < nsteps; k++) batch_launch(b, 1, -1, false);
         hipGraph_t gr = nullptr;
         BCHK(hipStreamEndCapture(b->s, &gr));
         b->graph = gr;
@@ -785,7 +820,7 @@ int dg_batch_run(dg_batch* b, int nsteps, int use_graph, double* device_ms) {
     if (use_graph) {
         BCHK(hipGraphLaunch(b->gexec, b->s));
     } else {
-        for (int k = 0; k < nsteps; k++) batch_launch(b, b->d_order, b->npatch, 1, 0);
+        for (int k = 0; k < nsteps; k++) batch_launch(b, 1, -1, false);
         BCHK(hipGetLastError());
     }
     if (device_ms) {
@@ -829,9 +864,11 @@ int dg_batch_set_tiles(dg_batch* b, const int* chunk, int n) {
     if (!b || !chunk || n < 1 || n > BATCH_CLASSES) return DG_ERR_ARG;
     for (int c = 0; c < n; c++)
         if (chunk[c] < 1 || chunk[c] > 4096) return DG_ERR_ARG;
+    BCHK(hipStreamSynchronize(b->s));
     for (int c = 0; c < n; c++) b->chunk[c] = chunk[c];
     batch_drop_graph(b);
-    return DG_OK;
+    int rc = batch_table(b, BK_PRED_X);
+    return rc ? rc : batch_table(b, BK_FLUX_X);
 }
 
 int dg_batch_get_tiles(const dg_batch* b, int* chunk, int n) {
@@ -842,20 +879,22 @@ int dg_batch_get_tiles(const dg_batch* b, int* chunk, int n) {
 
 int dg_batch_autotune(dg_batch* b, int reps, int* chunk_out, double* class_ms) {
     if (!b || reps < 1) return DG_ERR_ARG;
-    if (!kPersistentBatchEnabled) return batch_disabled(b);
     static const int cand[] = {2, 4, 8, 16, 32, 64, 128};
+    std::vector<int> ncls(BATCH_CLASSES, 0);
+    for (const auto& q : b->P) ncls[q.cls]++;
+    BCHK(hipStreamSynchronize(b->s));
+    batch_drop_graph(b);
     for (int c = 0; c < BATCH_CLASSES; c++) {
-        const int n = (int)b->cls_order[c].size();
         if (class_ms) class_ms[c] = 0;
-        if (!n) continue;
-        const int* ord = b->d_order + b->cls_off[c];
+        if (!ncls[c]) continue;
         double best = 1e300;
         int pick = b->chunk[c];
         for (int k : cand) {
             b->chunk[c] = k;
-            batch_launch(b, ord, n, 0, 0);  // warm, no commit
+            if (batch_table(b, BK_PRED_X) || batch_table(b, BK_FLUX_X)) return DG_ERR_HIP;
+            batch_launch(b, 0, c, false);  // warm; no commit: states and counters untouched
             BCHK(hipEventRecord(b->e0, b->s));
-            for (int r = 0; r < reps; r++) batch_launch(b, ord, n, 0, 0);
+            for (int r = 0; r < reps; r++) batch_launch(b, 0, c, false);
             BCHK(hipEventRecord(b->e1, b->s));
             BCHK(hipEventSynchronize(b->e1));
             float ms = 0;
@@ -865,9 +904,9 @@ int dg_batch_autotune(dg_batch* b, int reps, int* chunk_out, double* class_ms) {
         b->chunk[c] = pick;
         if (class_ms) class_ms[c] = best / reps;
     }
+    if (batch_table(b, BK_PRED_X) || batch_table(b, BK_FLUX_X)) return DG_ERR_HIP;
     if (chunk_out)
         for (int c = 0; c < BATCH_CLASSES; c++) chunk_out[c] = b->chunk[c];
-    batch_drop_graph(b);
     return DG_OK;
 }
 

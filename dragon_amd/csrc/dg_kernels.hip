@@ -1363,217 +1363,245 @@ void ctu_hydro3(const StepCtx& c, hipStream_t s) {
 }
 
 // ===================================================== batched patches
-// One workgroup advances one 2D MHD patch through the whole CTU + CT step:
-// boundary passes, CFL reduction, predictor sweeps, corner EMF, half-step
-// faces and state, corrector sweeps, EMF, Faraday, apply.  These are the
-// single-grid kernels' bodies run as workgroup-strided loops with a barrier
-// between stages.  A rejected attempt halves the patch's dt and repeats on
-// the device, as Grid::advance does per grid (CFL.cpp:151-213).  Workgroups
-// take patches from a queue, largest first; the last workgroup to finish
-// resets the queue, so one launch is one step of every patch and the launch
-// can be replayed from a graph.
-__device__ __forceinline__ void bsync() { __syncthreads(); }
-
-template <class R>
-__device__ void batch_patch(const BatchArgs& A, int p, double* s_red, unsigned long long* s_key) {
+// Many independent 2D MHD patches (CTU + CT) advanced stage by stage: each
+// launch below covers every active patch through a block table, and runs the
+// same kernel body as the single-grid step over the patch's own arrays.  The
+// step is begin -> boundary passes -> CFL -> dt -> predictor sweeps -> EMF ->
+// half-step faces -> half-step state -> corrector sweeps -> EMF -> Faraday ->
+// apply -> commit, a fixed sequence a graph can replay.  A patch whose
+// attempt fails keeps its state and retries at half the dt in the next step
+// (the dt halving of Grid::advance, CFL.cpp:151-213, one attempt per step).
+struct PatchView {
+    Ext ce, fe;
+    Soa W, Wn;
+    double *B[3], *Bn[3], *Bh[3], *E0[3], *Eh[3];
+    FSoa Fp[3], F[3];
+    double dx, dy;
+    int cls;
+};
+__device__ __forceinline__ PatchView patch_view(const BatchArgs& A, int p) {
     const PatchDesc P = A.P[p];
     constexpr int G = 3;  // validGhosts for MHD + CTU
-    const Ext ce = make_ext(2, P.n[0], P.n[1], 1, G);
-    const Ext fe = make_ext(2, P.n[0] + 1, P.n[1] + 1, 1, G);
+    PatchView v;
+    v.ce = make_ext(2, P.n[0], P.n[1], 1, G);
+    v.fe = make_ext(2, P.n[0] + 1, P.n[1] + 1, 1, G);
     const int par = A.parity[p];
-    const Soa W{A.W[par] + 8 * P.oc, P.nc}, Wn{A.W[1 - par] + 8 * P.oc, P.nc};
-    double *B[3], *Bn[3], *Bh[3], *E0[3], *Eh[3];
+    v.W = Soa{A.W[par] + 8 * P.oc, P.nc};
+    v.Wn = Soa{A.W[1 - par] + 8 * P.oc, P.nc};
     for (int c = 0; c < 3; c++) {
-        B[c] = A.B[par] + 3 * P.of + c * P.nfc;
-        Bn[c] = A.B[1 - par] + 3 * P.of + c * P.nfc;
-        Bh[c] = A.Bh + 3 * P.of + c * P.nfc;
-        E0[c] = A.E0 + 3 * P.of + c * P.nfc;
-        Eh[c] = A.Eh + 3 * P.of + c * P.nfc;
+        v.B[c] = A.B[par] + 3 * P.of + c * P.nfc;
+        v.Bn[c] = A.B[1 - par] + 3 * P.of + c * P.nfc;
+        v.Bh[c] = A.Bh + 3 * P.of + c * P.nfc;
+        v.E0[c] = A.E0 + 3 * P.of + c * P.nfc;
+        v.Eh[c] = A.Eh + 3 * P.of + c * P.nfc;
     }
-    FSoa Fp[3], F[3];  // flux pools hold FT elements at the same per-patch offsets
-    for (int ax = 0; ax < 2; ax++) {
-        Fp[ax] = FSoa{reinterpret_cast<FT*>(A.Fp) + 16 * P.oc + ax * 8 * P.nc, P.nc};
-        F[ax] = FSoa{reinterpret_cast<FT*>(A.F) + 16 * P.oc + ax * 8 * P.nc, P.nc};
+    for (int ax = 0; ax < 2; ax++) {  // flux pools hold FT elements at the same per-patch offsets
+        v.Fp[ax] = FSoa{reinterpret_cast<FT*>(A.Fp) + 16 * P.oc + ax * 8 * P.nc, P.nc};
+        v.F[ax] = FSoa{reinterpret_cast<FT*>(A.F) + 16 * P.oc + ax * 8 * P.nc, P.nc};
     }
-    Fp[2] = Fp[0];  // no z axis in 2D; never read
-    F[2] = F[0];
-    unsigned long long* err = A.err + p;
-    const int tx = threadIdx.x, nt = blockDim.x;
+    v.Fp[2] = v.Fp[0];  // no z axis in 2D; never read
+    v.F[2] = v.F[0];
+    v.dx = P.d[0];
+    v.dy = P.d[1];
+    v.cls = P.cls;
+    return v;
+}
+// workgroup -> (patch, work item); inactive patches' workgroups leave at once
+#define BATCH_ITEM(bt)                          \
+    const int p = bt.patch[blockIdx.x];         \
+    if (!A.active[p]) return;                   \
+    const long t = (long)bt.first[blockIdx.x] + threadIdx.x;
 
-    // boundary: cell passes by axis and side, then face passes (the order of
-    // one list element's apply, corner ghosts on)
-    {
-        BCArgs b{};
-        b.W = W;
-        for (int c = 0; c < 3; c++) b.B[c] = B[c];
-        b.ce = ce; b.fe = fe; b.dim = 2; b.nf = 8; b.kind = A.bc_kind; b.corners = 1; b.gated = 0;
-        const int s0 = A.bc_kind == 1 ? 2 : 0, s1 = A.bc_kind == 1 ? 2 : 1;
-        for (int pass = 0; pass < 2; pass++)
-            for (int ax = 0; ax < 2; ax++) {
-                b.axis = ax;
-                b.lo[0] = -G; b.hi[0] = ce.n[ax == 0 ? 1 : 0] + G + pass;
-                b.lo[1] = 0; b.hi[1] = 1;
-                const long cnt = b.hi[0] - b.lo[0];
-                for (int side = s0; side <= s1; side++) {
-                    b.side = side;
-                    for (long t = tx; t < cnt; t += nt) {
-                        if (pass == 0) bc_cells_body(b, t);
-                        else bc_faces_body(b, t);
-                    }
-                    bsync();
-                }
-            }
+KB __global__ void kb_begin(BatchArgs A) {
+    const int p = (int)tid();
+    if (p >= A.npatch) return;
+    const bool on = A.status[p] == 0 && (A.cls_filter < 0 || A.P[p].cls == A.cls_filter);
+    A.active[p] = on ? 1 : 0;
+    if (on) {
+        A.vmax[p] = 0ull;
+        A.err[p] = ~0ull;
     }
-    if (A.init_only) {  // initialize_B_fields: cell B = face average everywhere, p held
-        for (long m = tx; m < ce.size; m += nt) {
-            const int i = (int)(m / (ce.n[1] + 2 * G)) - G, j = (int)(m % (ce.n[1] + 2 * G)) - G;
-            double x, y, z;
-            cell_B(B[0], B[1], B[2], fe, 2, i, j, 0, x, y, z);
-            W(F_BX, m) = x; W(F_BY, m) = y; W(F_BZ, m) = z;
-        }
+}
+// Boundary pass of every patch: both sides of one axis in one launch (they
+// touch disjoint ghost slabs for sides of 4 cells or more), in the pass
+// order of a list element's apply.
+template <bool FACES, int AX>
+KB __global__ void kb_bc(BatchArgs A, BlockTable bt) {
+    BATCH_ITEM(bt)
+    const PatchView v = patch_view(A, p);
+    constexpr int G = 3;
+    const long cnt = v.ce.n[AX == 0 ? 1 : 0] + 2 * G + (FACES ? 1 : 0);
+    const bool periodic = A.bc_kind == 1;
+    if (t >= (periodic ? 1 : 2) * cnt) return;
+    BCArgs b{};
+    b.W = v.W;
+    for (int c = 0; c < 3; c++) b.B[c] = v.B[c];
+    b.ce = v.ce; b.fe = v.fe; b.dim = 2; b.nf = 8; b.kind = A.bc_kind; b.corners = 1; b.gated = 0;
+    b.axis = AX;
+    b.lo[0] = -G; b.hi[0] = (int)(-G + cnt);
+    b.lo[1] = 0; b.hi[1] = 1;
+    b.side = periodic ? 2 : (int)(t / cnt);
+    if (FACES) bc_faces_body(b, t % cnt);
+    else bc_cells_body(b, t % cnt);
+}
+KB __global__ void kb_init_B(BatchArgs A, BlockTable bt) {
+    BATCH_ITEM(bt)
+    const PatchView v = patch_view(A, p);
+    if (t >= v.ce.size) return;
+    const int ny = v.ce.n[1] + 6;
+    const int i = (int)(t / ny) - 3, j = (int)(t % ny) - 3;
+    double x, y, z;
+    cell_B(v.B[0], v.B[1], v.B[2], v.fe, 2, i, j, 0, x, y, z);
+    v.W(F_BX, t) = x; v.W(F_BY, t) = y; v.W(F_BZ, t) = z;
+}
+KB __global__ void kb_cfl(BatchArgs A, BlockTable bt) {
+    BATCH_ITEM(bt)
+    const PatchView v = patch_view(A, p);
+    const long ny2 = v.ce.n[1] + 2;
+    double m = 0;
+    if (t < (long)(v.ce.n[0] + 2) * ny2)
+        m = dmax(m, cfl_speed_at<2, true>(v.W, v.ce, (int)(t / ny2) - 1, (int)(t % ny2) - 1, 0, A.ph, A.cfl_mode, v.dx, v.dy, 1.0));
+    for (int off = 32; off > 0; off >>= 1) m = dmax(m, __shfl_xor(m, off));
+    __shared__ double wmax[BATCH_THREADS / 64];
+    if ((threadIdx.x & 63) == 0) wmax[threadIdx.x >> 6] = m;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double b = wmax[0];
+        for (int q = 1; q < BATCH_THREADS / 64; q++) b = dmax(b, wmax[q]);
+        if (b > 0) atomicMax(A.vmax + p, (unsigned long long)__double_as_longlong(b));
+    }
+}
+KB __global__ void kb_dt(BatchArgs A) {
+    const int p = (int)tid();
+    if (p >= A.npatch || !A.active[p]) return;
+    const double m = __longlong_as_double((long long)A.vmax[p]);
+    if (!(m > 0)) {  // "CFL timestep failed: max signal speed is zero"
+        A.active[p] = 0;
+        if (A.commit) A.status[p] = BATCH_ERR_CFL;
         return;
     }
-
-    // CFL: max signal speed over cells [-1, n+1)^2 (NaN never wins)
-    double dt;
-    {
-        double m = 0;
-        const long ny2 = ce.n[1] + 2, cnt = (long)(ce.n[0] + 2) * ny2;
-        for (long t = tx; t < cnt; t += nt)
-            m = dmax(m, cfl_speed_at<2, true>(W, ce, (int)(t / ny2) - 1, (int)(t % ny2) - 1, 0, A.ph, A.cfl_mode, P.d[0], P.d[1], 1.0));
-        for (int off = 32; off > 0; off >>= 1) m = dmax(m, __shfl_xor(m, off));
-        if ((tx & 63) == 0) s_red[tx >> 6] = m;
-        bsync();
-        m = s_red[0];
-        for (int q = 1; q < (nt >> 6); q++) m = dmax(m, s_red[q]);
-        bsync();
-        if (!(m > 0)) {
-            if (tx == 0 && A.commit) A.status[p] = BATCH_ERR_CFL;
-            return;
-        }
-        dt = A.cfl_coeff / m;
+    double dt = A.cfl_coeff / m;
+    for (int h = 0; h < A.halv[p]; h++) dt *= 0.5;
+    A.dt[p] = dt;
+}
+template <int AX, bool FIN>
+__device__ __forceinline__ SweepArgs batch_sweep_args(const BatchArgs& A, const PatchView& v, int p) {
+    SweepArgs s{};
+    const double dt = A.dt[p];
+    const double dt_d[3] = {dt / v.dx, dt / v.dy, dt / 1.0};
+    s.W = v.W;
+    for (int q = 0; q < 3; q++) {
+        s.B[q] = v.B[q]; s.Bh[q] = v.Bh[q]; s.E0[q] = v.E0[q]; s.Fp[q] = v.Fp[q]; s.dt_d[q] = dt_d[q];
     }
-
-    const int chunk = A.chunk[P.cls];
-    long nfail = 0;
-    while (true) {
-        if (tx == 0) atomicExch(err, ~0ull);
-        __threadfence();
-        bsync();
-        const double dt_d[3] = {dt / P.d[0], dt / P.d[1], dt / 1.0};
-        SweepArgs s{};
-        s.W = W;
-        for (int q = 0; q < 3; q++) {
-            s.B[q] = B[q]; s.Bh[q] = Bh[q]; s.E0[q] = E0[q]; s.Fp[q] = Fp[q]; s.dt_d[q] = dt_d[q];
-        }
-        s.ce = ce; s.fe = fe; s.ph = A.ph; s.err = err;
-        auto box = [&](int ax, int alo, int ahi, int tlo, int thi) {
-            const int t = ax == 0 ? 1 : 0;
-            s.lo[ax] = alo; s.hi[ax] = ahi; s.lo[t] = tlo; s.hi[t] = thi; s.lo[2] = 0; s.hi[2] = 1;
-        };
-        auto march = [&](auto fin) {  // x faces, strided axis: chunk faces per thread
-            constexpr bool FIN = decltype(fin)::value;
-            const int len = s.hi[0] - s.lo[0];
-            s.chunk = chunk < len ? chunk : len;
-            const long items = (long)((len + s.chunk - 1) / s.chunk) * (s.hi[1] - s.lo[1]);
-            for (long t = tx; t < items; t += nt) sweep_march_body<2, true, R, 0, FIN>(s, t);
-            bsync();
-        };
-        auto wave = [&](auto fin) {  // y faces, contiguous axis: 63 faces per wavefront
-            constexpr bool FIN = decltype(fin)::value;
-            s.chunk = 63;
-            const long waves = (long)((s.hi[1] - s.lo[1] + 62) / 63) * (s.hi[0] - s.lo[0]);
-            for (long w = tx >> 6; w < waves; w += nt >> 6) sweep_wave_body<2, true, R, 1, FIN>(s, w, tx & 63);
-            bsync();
-        };
-        // predictor fluxes on [-1, n+1] x [-2, n+2)
-        box(0, -1, ce.n[0] + 2, -2, ce.n[1] + 2); s.Fo = Fp[0]; s.stage = ST_PRED_X;
-        march(std::false_type{});
-        box(1, -1, ce.n[1] + 2, -2, ce.n[0] + 2); s.Fo = Fp[1]; s.stage = ST_PRED_Y;
-        wave(std::false_type{});
-        {
-            const ERef none{nullptr, nullptr, nullptr};
-            const Box eb = make_box(2, -1, ce.n[0] + 2, -1, ce.n[1] + 2);
-            for (long t = tx; t < eb.count(); t += nt) emf2_body<true>(Fp[0], Fp[1], W, none, ce, fe, eb, 1, E0[0], E0[1], E0[2], t);
-            bsync();
-            for (long m = tx; m < fe.size; m += nt)
-                faraday_body<2>(E0[0], E0[1], E0[2], B[0], B[1], B[2], Bh[0], Bh[1], Bh[2], fe, 1, 0.5 * dt_d[0], 0.5 * dt_d[1],
-                                0.5 * dt_d[2], m);
-            bsync();
-            const Box cb = make_box(2, -1, ce.n[0] + 1, -1, ce.n[1] + 1);
-            for (long t = tx; t < cb.count(); t += nt)
-                whalf_body<2>(W, Fp[0], Fp[1], Fp[2], Bh[0], Bh[1], Bh[2], ce, fe, cb, 0.5 * dt_d[0], 0.5 * dt_d[1], 0.5 * dt_d[2], A.ph,
-                              Eh[0], Eh[1], Eh[2], err, t);
-            bsync();
-        }
-        // corrector fluxes on [0, n] x [-1, n+1), CTU correction inside
-        box(0, 0, ce.n[0] + 1, -1, ce.n[1] + 1); s.Fo = F[0]; s.stage = ST_FLUX_X;
-        march(std::true_type{});
-        box(1, 0, ce.n[1] + 1, -1, ce.n[0] + 1); s.Fo = F[1]; s.stage = ST_FLUX_Y;
-        wave(std::true_type{});
-        {
-            const ERef eh{Eh[0], Eh[1], Eh[2]};
-            const Box eb = make_box(2, 0, ce.n[0] + 1, 0, ce.n[1] + 1);
-            for (long t = tx; t < eb.count(); t += nt) emf2_body<false>(F[0], F[1], W, eh, ce, fe, eb, 1, E0[0], E0[1], E0[2], t);
-            bsync();
-            for (long m = tx; m < fe.size; m += nt)
-                faraday_body<2>(E0[0], E0[1], E0[2], B[0], B[1], B[2], Bn[0], Bn[1], Bn[2], fe, 0, dt_d[0], dt_d[1], dt_d[2], m);
-            bsync();
-            for (long m = tx; m < ce.size; m += nt)
-                apply_body<2, true>(W, Wn, F[0], F[1], F[2], Bn[0], Bn[1], Bn[2], ce, fe, dt_d[0], dt_d[1], dt_d[2], A.ph, err, m);
-        }
-        __threadfence();
-        bsync();
-        if (tx == 0) *s_key = atomicOr(err, 0ull);
-        bsync();
-        const unsigned long long key = *s_key;
-        if (key == ~0ull) break;
-        nfail++;
-        dt *= 0.5;
-        if (dt < A.tol) {
-            if (tx == 0 && A.commit) { A.status[p] = BATCH_ERR_DT; A.fails[p] += nfail; }
-            return;
-        }
+    s.ce = v.ce; s.fe = v.fe; s.ph = A.ph; s.err = A.err + p;
+    const int nx = v.ce.n[0], ny = v.ce.n[1];
+    constexpr int T = AX == 0 ? 1 : 0;
+    // predictor faces [-1, n+1] x [-2, n+2); corrector faces [0, n] x [-1, n+1)
+    s.lo[AX] = FIN ? 0 : -1;
+    s.hi[AX] = (AX == 0 ? nx : ny) + (FIN ? 1 : 2);
+    s.lo[T] = FIN ? -1 : -2;
+    s.hi[T] = (T == 0 ? nx : ny) + (FIN ? 1 : 2);
+    s.lo[2] = 0; s.hi[2] = 1;
+    s.Fo = FIN ? v.F[AX] : v.Fp[AX];
+    s.stage = (FIN ? ST_FLUX_X : ST_PRED_X) + AX;
+    return s;
+}
+template <bool FIN>
+KS __global__ void kb_sweep_x(BatchArgs A, BlockTable bt) {
+    BATCH_ITEM(bt)
+    const PatchView v = patch_view(A, p);
+    SweepArgs s = batch_sweep_args<0, FIN>(A, v, p);
+    const int len = s.hi[0] - s.lo[0];
+    s.chunk = A.chunk[v.cls] < len ? A.chunk[v.cls] : len;
+    sweep_march_body<2, true, RT, 0, FIN>(s, t);
+}
+template <bool FIN>
+KS __global__ void kb_sweep_y(BatchArgs A, BlockTable bt) {
+    BATCH_ITEM(bt)  // first is a multiple of the workgroup size, so t >> 6 is this wavefront
+    const PatchView v = patch_view(A, p);
+    SweepArgs s = batch_sweep_args<1, FIN>(A, v, p);
+    s.chunk = 63;
+    sweep_wave_body<2, true, RT, 1, FIN>(s, t >> 6, threadIdx.x & 63);
+}
+template <bool PRED>
+KB __global__ void kb_emf(BatchArgs A, BlockTable bt) {
+    BATCH_ITEM(bt)
+    const PatchView v = patch_view(A, p);
+    const int lo = PRED ? -1 : 0, ex = PRED ? 2 : 1;
+    const Box eb = make_box(2, lo, v.ce.n[0] + ex, lo, v.ce.n[1] + ex);
+    if (PRED) {
+        const ERef none{nullptr, nullptr, nullptr};
+        emf2_body<true>(v.Fp[0], v.Fp[1], v.W, none, v.ce, v.fe, eb, 1, v.E0[0], v.E0[1], v.E0[2], t);
+    } else {
+        const ERef eh{v.Eh[0], v.Eh[1], v.Eh[2]};
+        emf2_body<false>(v.F[0], v.F[1], v.W, eh, v.ce, v.fe, eb, 1, v.E0[0], v.E0[1], v.E0[2], t);
     }
-    if (tx == 0 && A.commit) {
-        A.parity[p] = 1 - par;
-        A.dt_last[p] = dt;
+}
+template <bool HALF>
+KB __global__ void kb_faraday(BatchArgs A, BlockTable bt) {
+    BATCH_ITEM(bt)
+    const PatchView v = patch_view(A, p);
+    const double dt = A.dt[p], f = HALF ? 0.5 : 1.0;
+    const double dx = f * (dt / v.dx), dy = f * (dt / v.dy), dz = f * (dt / 1.0);
+    if (HALF) faraday_body<2>(v.E0[0], v.E0[1], v.E0[2], v.B[0], v.B[1], v.B[2], v.Bh[0], v.Bh[1], v.Bh[2], v.fe, 1, dx, dy, dz, t);
+    else faraday_body<2>(v.E0[0], v.E0[1], v.E0[2], v.B[0], v.B[1], v.B[2], v.Bn[0], v.Bn[1], v.Bn[2], v.fe, 0, dx, dy, dz, t);
+}
+KB __global__ void kb_whalf(BatchArgs A, BlockTable bt) {
+    BATCH_ITEM(bt)
+    const PatchView v = patch_view(A, p);
+    const double dt = A.dt[p];
+    const Box cb = make_box(2, -1, v.ce.n[0] + 1, -1, v.ce.n[1] + 1);
+    whalf_body<2>(v.W, v.Fp[0], v.Fp[1], v.Fp[2], v.Bh[0], v.Bh[1], v.Bh[2], v.ce, v.fe, cb, 0.5 * (dt / v.dx), 0.5 * (dt / v.dy),
+                  0.5 * (dt / 1.0), A.ph, v.Eh[0], v.Eh[1], v.Eh[2], A.err + p, t);
+}
+KB __global__ void kb_apply(BatchArgs A, BlockTable bt) {
+    BATCH_ITEM(bt)
+    const PatchView v = patch_view(A, p);
+    const double dt = A.dt[p];
+    apply_body<2, true>(v.W, v.Wn, v.F[0], v.F[1], v.F[2], v.Bn[0], v.Bn[1], v.Bn[2], v.ce, v.fe, dt / v.dx, dt / v.dy, dt / 1.0, A.ph,
+                        A.err + p, t);
+}
+KB __global__ void kb_commit(BatchArgs A) {
+    const int p = (int)tid();
+    if (p >= A.npatch || !A.active[p] || !A.commit) return;
+    if (A.err[p] == ~0ull) {
+        A.parity[p] = 1 - A.parity[p];
+        A.dt_last[p] = A.dt[p];
         A.steps[p] += 1;
-        A.fails[p] += nfail;
-        A.status[p] = 0;
+        A.halv[p] = 0;
+    } else {
+        A.fails[p] += 1;
+        A.halv[p] += 1;
+        if (0.5 * A.dt[p] < A.tol) A.status[p] = BATCH_ERR_DT;  // "Timestep has fallen below minimum"
     }
 }
 
-template <class R>
-__global__ __launch_bounds__(BATCH_THREADS) void k_batch(BatchArgs A) {
-    __shared__ double s_red[BATCH_THREADS / 64];
-    __shared__ unsigned long long s_key;
-    __shared__ int s_next;
-    while (true) {
-        if (threadIdx.x == 0) s_next = (int)atomicAdd(&A.queue[0], 1u);
-        __syncthreads();
-        const int q = s_next;
-        __syncthreads();
-        if (q >= A.npatch) break;
-        batch_patch<R>(A, A.order[q], s_red, &s_key);
-        __syncthreads();
+void launch_batch_step(const BatchArgs& a, const BlockTable* bt, bool init_only, hipStream_t s) {
+    const dim3 T(BATCH_THREADS);
+    const unsigned pb = blocks(a.npatch);
+    auto grid = [&](int k) { return dim3(bt[k].nblocks); };
+    hipLaunchKernelGGL(kb_begin, dim3(pb), T, 0, s, a);
+    hipLaunchKernelGGL((kb_bc<false, 0>), grid(BK_BCC_X), T, 0, s, a, bt[BK_BCC_X]);
+    hipLaunchKernelGGL((kb_bc<false, 1>), grid(BK_BCC_Y), T, 0, s, a, bt[BK_BCC_Y]);
+    hipLaunchKernelGGL((kb_bc<true, 0>), grid(BK_BCF_X), T, 0, s, a, bt[BK_BCF_X]);
+    hipLaunchKernelGGL((kb_bc<true, 1>), grid(BK_BCF_Y), T, 0, s, a, bt[BK_BCF_Y]);
+    if (init_only) {
+        hipLaunchKernelGGL(kb_init_B, grid(BK_CELLS), T, 0, s, a, bt[BK_CELLS]);
+        return;
     }
-    if (threadIdx.x == 0) {
-        __threadfence();
-        if (atomicAdd(&A.queue[1], 1u) == gridDim.x - 1) {
-            atomicExch(&A.queue[0], 0u);
-            atomicExch(&A.queue[1], 0u);
-        }
-    }
-}
-
-void launch_batch(const BatchArgs& a, int nblocks, hipStream_t s) {
-    hipLaunchKernelGGL((k_batch<RT>), dim3(nblocks), dim3(BATCH_THREADS), 0, s, a);
-}
-int batch_blocks_per_cu() {
-    int n = 0;
-    hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_batch<RT>, BATCH_THREADS, 0);
-    return e == hipSuccess && n > 0 ? n : 1;
+    hipLaunchKernelGGL(kb_cfl, grid(BK_CELLS1), T, 0, s, a, bt[BK_CELLS1]);
+    hipLaunchKernelGGL(kb_dt, dim3(pb), T, 0, s, a);
+    hipLaunchKernelGGL((kb_sweep_x<false>), grid(BK_PRED_X), T, 0, s, a, bt[BK_PRED_X]);
+    hipLaunchKernelGGL((kb_sweep_y<false>), grid(BK_PRED_Y), T, 0, s, a, bt[BK_PRED_Y]);
+    hipLaunchKernelGGL((kb_emf<true>), grid(BK_EDGES_P), T, 0, s, a, bt[BK_EDGES_P]);
+    hipLaunchKernelGGL((kb_faraday<true>), grid(BK_FACES), T, 0, s, a, bt[BK_FACES]);
+    hipLaunchKernelGGL(kb_whalf, grid(BK_CELLS1), T, 0, s, a, bt[BK_CELLS1]);
+    hipLaunchKernelGGL((kb_sweep_x<true>), grid(BK_FLUX_X), T, 0, s, a, bt[BK_FLUX_X]);
+    hipLaunchKernelGGL((kb_sweep_y<true>), grid(BK_FLUX_Y), T, 0, s, a, bt[BK_FLUX_Y]);
+    hipLaunchKernelGGL((kb_emf<false>), grid(BK_EDGES), T, 0, s, a, bt[BK_EDGES]);
+    hipLaunchKernelGGL((kb_faraday<false>), grid(BK_FACES), T, 0, s, a, bt[BK_FACES]);
+    hipLaunchKernelGGL(kb_apply, grid(BK_CELLS), T, 0, s, a, bt[BK_CELLS]);
+    hipLaunchKernelGGL(kb_commit, dim3(pb), T, 0, s, a);
 }
 
 void launch_step(const StepCtx& c, hipStream_t s) {

@@ -55,9 +55,11 @@ struct BCArgs {
     double st[8];
     int lo[2], hi[2];                           // transverse ranges
 };
-// Batched 2D MHD patches (CTU + CT), one workgroup per patch per step.
-// Pools hold every patch back to back: patch p's cell field f of pool X is
-// at X + nfield*oc + f*nc; face component c at X + 3*of + c*nfc.
+// Batched 2D MHD patches (CTU + CT).  Every stage of the step is one launch
+// over all patches: a block table per work kind maps each workgroup to a
+// patch and the first work item it covers.  Pools hold every patch back to
+// back: patch p's cell field f of pool X is at X + nfield*oc + f*nc; face
+// component c at X + 3*of + c*nfc.
 struct PatchDesc {
     int n[2];           // interior cells
     double d[2];        // dx, dy
@@ -68,9 +70,25 @@ struct PatchDesc {
 enum { BATCH_CLASSES = 4, BATCH_THREADS = 256 };
 // per-patch status codes (the values of DG_ERR_CFL / DG_ERR_DT_UNDERFLOW)
 enum { BATCH_ERR_CFL = 5, BATCH_ERR_DT = 6 };
+// work kinds of a batched step (items per patch in dg_api.hip: batch_items)
+enum BatchKind {
+    BK_BCC_X, BK_BCC_Y, BK_BCF_X, BK_BCF_Y,  // boundary passes: cells / faces, x then y, both sides
+    BK_CELLS1,                               // cells [-1, n+1)^2: CFL, half-step state
+    BK_PRED_X, BK_PRED_Y,                    // predictor sweeps (x: faces-per-thread chunks; y: 63 faces per wavefront)
+    BK_EDGES_P,                              // edges [-1, n+2)^2: predictor EMF
+    BK_FACES,                                // every face: Faraday
+    BK_FLUX_X, BK_FLUX_Y,                    // corrector sweeps
+    BK_EDGES,                                // edges [0, n+1)^2: EMF
+    BK_CELLS,                                // every cell: apply, initialize_B_fields
+    BK_COUNT
+};
+struct BlockTable {
+    const int* patch;   // per workgroup: patch
+    const int* first;   // per workgroup: first work item (a multiple of BATCH_THREADS)
+    unsigned nblocks;
+};
 struct BatchArgs {
     const PatchDesc* P;
-    const int* order;          // processing order: largest patches first
     int npatch;
     double* W[2];              // cell state pools (8 fields), double-buffered
     double* B[2];              // face-B pools, double-buffered
@@ -82,17 +100,21 @@ struct BatchArgs {
     int* parity;               // per patch: index of the current buffer
     double* dt_last;           // per patch: dt of the last accepted step
     long* steps;               // per patch: accepted steps
-    long* fails;               // per patch: rejected attempts (dt halvings)
-    int* status;               // per patch: 0, or the DG_* code of the last step that could not complete
+    long* fails;               // per patch: rejected attempts
+    int* status;               // per patch: 0, or the DG_* code that stopped it
     unsigned long long* err;   // per patch failure key of the current attempt
-    unsigned* queue;           // [0] next index into order, [1] finished workgroups
+    unsigned long long* vmax;  // per patch CFL signal speed (FP64 bits)
+    double* dt;                // per patch dt of the current attempt
+    int* halv;                 // per patch halvings since the last accepted step
+    int* active;               // per patch: steps in this launch sequence
     Phys ph;
     double cfl_coeff, tol;
     int cfl_mode, bc_kind;     // bc_kind: 0 outflow, 1 periodic (all faces, corner ghosts)
     int chunk[BATCH_CLASSES];  // x-sweep faces per thread, per size class
-    int commit;                // 0: run the step but keep the old state (tuning)
-    int init_only;             // 1: boundary + cell B from faces (initialize_B_fields) only
+    int commit;                // 0: run the step, keep every patch's state and counters (tuning)
+    int cls_filter;            // -1: all patches, else only this size class
 };
+
 // The kernels are compiled twice (dg_kernels.hip): p64 is the FP64 drop-in
 // (no FMA contraction, correctly rounded division and square root); p32 holds
 // the mixed form's kernels (FP32 reconstruction and Riemann solves with
@@ -100,8 +122,7 @@ struct BatchArgs {
 // pressure recovery and CT).  Separate namespaces keep every kernel symbol of
 // one build out of the other.
 #define DG_LAUNCH_INTERFACE                                                                                     \
-    void launch_batch(const BatchArgs& a, int blocks, hipStream_t s);                                            \
-    int batch_blocks_per_cu();                                                                                  \
+    void launch_batch_step(const BatchArgs& a, const BlockTable* bt, bool init_only, hipStream_t s);           \
     void launch_step(const StepCtx& c, hipStream_t s);                                                          \
     void launch_bc(const BCArgs& a, bool faces, hipStream_t s);                                                 \
     void launch_cfl(const StepCtx& c, unsigned long long* out, hipStream_t s);                                  \

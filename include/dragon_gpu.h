@@ -145,10 +145,13 @@ int dg_divb(dg_grid* g, double* out);
  * Batched patches.  Replaces the per-child dispatch of DistGrid<T>::step
  * (Refinement/DistGrid.cpp:291-303, children of DistGrid.hpp:20, each child
  * running Grid2D::unsplit_step on a DRAGONWING thread) for many independent
- * 2D MHD grids (CTU + CT, 3 ghosts, the physics switches of `physics`): one
- * persistent launch advances every patch by one step of its own CFL dt,
- * with the dt halving of Grid::advance (CFL.cpp:151-213) done per patch on
- * the device.  Unlike DistGrid children, patches do not share a dt or halos.
+ * 2D MHD grids (CTU + CT, 3 ghosts, the physics switches of `physics`).  A
+ * step is a fixed sequence of launches, each covering every patch; it
+ * attempts one step of each patch at the patch's own CFL dt.  The dt halving
+ * of Grid::advance (CFL.cpp:151-213) is kept per patch on the device: a
+ * patch whose attempt fails keeps its state and retries at half the dt on
+ * the next step (steps counts committed steps, fails rejected attempts).
+ * Unlike DistGrid children, patches do not share a dt or halos.
  * Boundaries: bc_kind DG_BC_OUTFLOW or DG_BC_PERIODIC on every face of every
  * patch, corner ghosts on.  Patch sides must be >= 4 cells.
  * --------------------------------------------------------------------- */
