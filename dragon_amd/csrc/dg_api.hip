@@ -32,6 +32,7 @@ struct dg_grid {
     double *B = nullptr, *Bn = nullptr, *Bh = nullptr;
     double *HS = nullptr, *F = nullptr, *Fp = nullptr, *T = nullptr;
     bool fused = true;
+    int march_chunk = 0;
     double *E0 = nullptr, *Eh = nullptr;
     double* cflpart = nullptr;  // per-block CFL maxima of the committed state, written by the apply pass
     long ncflpart = 0;
@@ -122,6 +123,7 @@ static StepCtx make_ctx(dg_grid* g, double dt) {
     c.dim = g->d.dim; c.mhd = g->d.mhd != 0; c.mixed = g->d.precision == DG_MIXED;
     c.ctu = g->d.ctu; c.cfl_mode = g->d.cfl_mode;
     c.fused = g->fused;
+    c.march_chunk = g->march_chunk;
     c.ce = g->ce; c.fe = g->fe; c.ph = g->ph;
     c.d[0] = g->d.dx; c.d[1] = g->d.dim >= 2 ? g->d.dy : 1.0; c.d[2] = g->d.dim >= 3 ? g->d.dz : 1.0;
     c.dt = dt;
@@ -182,6 +184,8 @@ int dg_create(const dg_desc* desc, dg_grid** out) {
     // 1D and 3D hydro keep the reference's half-state arrays
     const bool fusable = (d.mhd && d.dim >= 2) || (!d.mhd && d.dim == 2);
     g->fused = fusable && getenv("DG_UNFUSED") == nullptr;
+    // diagnostic: fixed faces per walker of the strided-axis sweeps (results do not depend on it)
+    if (const char* mc = getenv("DG_MARCH_CHUNK")) g->march_chunk = atoi(mc);
     ok = ok && alloc(&g->F, (long)d.dim * g->nf * nc);
     if (g->fused) ok = ok && (!d.ctu || alloc(&g->Fp, (long)d.dim * g->nf * nc));
     else ok = ok && alloc(&g->HS, 2L * d.dim * g->nf * nc);

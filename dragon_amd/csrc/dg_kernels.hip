@@ -274,7 +274,7 @@ KB __global__ void k_cfl_tail(Soa W, Ext ce, const double* part, long nparts, Ph
         } else {
             r -= 2 * (sx + sy);
             k = r < sz ? -1 : (int)n2;
-            r %= sz;
+            r %= (sz > 0 ? sz : 1);  // 3D only: the shell has no z faces below 3D
             i = (int)(r / n1); j = (int)(r % n1);
         }
         m = dmax(m, cfl_speed_at<D, MHD>(W, ce, i, j, k, ph, mode, dx, dy, dz));
@@ -1129,12 +1129,15 @@ static void sweep_t(const StepCtx& c, const Box& b, const FSoa& out, int stage, 
         a.chunk = 63;
         hipLaunchKernelGGL((k_sweep_wave<D, MHD, R, AX, FINAL>), dim3(blocks(waves * 64)), dim3(256), 0, s, a);
     } else {
-        // enough independent walkers to fill the chip (~2^18 threads), each
-        // walking at least 4 faces
-        const long want = 1L << 18;
+        // Walkers of at least 16 faces (one extra cell rebuild per walker),
+        // ~2^22 of them where the grid allows: with 2 resident waves per SIMD
+        // that is ~32 rounds of waves, so the last, partial round costs
+        // little.  (A walker per whole line, 4161 waves at 512^3, ran as two
+        // full rounds and a third of 65 waves.)
+        const long want = 1L << 22;
         long nch = (want + ntrans - 1) / ntrans;
-        long chunk = (len + nch - 1) / nch;
-        if (chunk < 4) chunk = 4;
+        long chunk = c.march_chunk > 0 ? c.march_chunk : (len + nch - 1) / nch;
+        if (c.march_chunk <= 0 && chunk < 16) chunk = 16;
         if (chunk > len) chunk = len;
         nch = (len + chunk - 1) / chunk;
         a.chunk = (int)chunk;

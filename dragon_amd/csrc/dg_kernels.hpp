@@ -23,6 +23,7 @@ struct StepCtx {
     bool mhd, mixed;
     int ctu, cfl_mode;
     bool fused;           // fused sweeps (no half-state arrays) where available
+    int march_chunk;      // faces one thread walks in a strided-axis sweep (0: automatic)
     Ext ce, fe;
     Phys ph;
     double d[3];
