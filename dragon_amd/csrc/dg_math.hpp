@@ -52,6 +52,12 @@ __device__ __forceinline__ bool fin(float a) { return __builtin_isfinite(a); }
 template <class T> struct P {
     T rho, p, vx, vy, vz, Bx, By, Bz;
 };
+// c ? a : b field by field (a select of whole records can end up as a
+// private-memory copy addressed by the condition)
+template <class T> __device__ __forceinline__ P<T> psel(bool c, const P<T>& a, const P<T>& b) {
+    return P<T>{c ? a.rho : b.rho, c ? a.p : b.p, c ? a.vx : b.vx, c ? a.vy : b.vy,
+                c ? a.vz : b.vz, c ? a.Bx : b.Bx, c ? a.By : b.By, c ? a.Bz : b.Bz};
+}
 // Conservative record / flux: rho, E, mom, B (FluidElement.hpp:70-80).
 template <class T> struct U {
     T rho, E, mx, my, mz, Bx, By, Bz;
@@ -312,7 +318,7 @@ template <class T> __device__ U<T> hllc(const P<T>& L, const P<T>& R, const Phys
     if (sr <= T(0)) return pflux<false>(R, ph);
     T pr = R.p + R.rho * R.vx * (R.vx - sr), pl = L.p + L.rho * L.vx * (L.vx - sl);
     T sc = (pr - pl) / (R.rho * (R.vx - sr) - L.rho * (L.vx - sl));
-    const P<T> X = sc > T(0) ? L : R;
+    const P<T> X = psel(sc > T(0), L, R);
     T sx = sc > T(0) ? sl : sr;
     U<T> UX = cons<false>(X, ph);
     U<T> Us = udiv<false>(umul<false>(UX, sx - X.vx), sx - sc);
@@ -366,15 +372,44 @@ template <class T> __device__ __forceinline__ P<T> star_prim(const U<T>& us, T p
     w.p = pT - dot3(w.Bx, w.By, w.Bz, w.Bx, w.By, w.Bz) * T(ph.k1_8pi);
     return w;
 }
-template <class T> __device__ U<T> hlld(P<T> L, P<T> R, const Phys& ph) {  // HLLD.cpp:38-133, :152-205
+// Floor of the zero-normal-B test's scale (HLLD.cpp:50-51 uses 1e-100).
+// 1e-100 is zero in FP32, which would send faces with B = 0 exactly past the
+// zero-B solver; the FP32 form uses 1e-30 so they take the same branch.
+template <class T> __device__ __forceinline__ T hlld_floor() { return sizeof(T) == 4 ? T(1e-30) : T(1e-100); }
+template <bool MHD, class T> __device__ __forceinline__ U<T> usel(bool c, const U<T>& a, const U<T>& b) {
+    U<T> r;
+    r.rho = c ? a.rho : b.rho; r.E = c ? a.E : b.E; r.mx = c ? a.mx : b.mx; r.my = c ? a.my : b.my; r.mz = c ? a.mz : b.mz;
+    if constexpr (MHD) { r.Bx = c ? a.Bx : b.Bx; r.By = c ? a.By : b.By; r.Bz = c ? a.Bz : b.Bz; }
+    else { r.Bx = r.By = r.Bz = T(0); }
+    return r;
+}
+// Outer-star flux F_K + S_K (U*_K - U_K) with the normal B flux zeroed.
+template <class T> __device__ __forceinline__ U<T> star_flux(const P<T>& K, const U<T>& usK, T SK, const Phys& ph) {
+    U<T> f = uadd<true>(pflux<true>(K, ph), umul<true>(usub<true>(usK, cons<true>(K, ph)), SK));
+    f.Bx = 0;
+    return f;
+}
+// HLLD for a wavefront.  The reference's solver (HLLD.cpp) branches on the
+// wave region of the face: supersonic left/right, outer star left/right,
+// double star.  In a wavefront whose faces fall in different regions every
+// branch would run in turn.  Here every face of the nonzero-normal-B solver
+// follows one instruction stream: both outer star states and the double-star
+// state are built, the upwind side K is chosen per face, and the result is
+// selected from F(K), F*K = F(K) + S_K (U*K - U_K) and F*K + S*K (U**K - U*K).
+// Each face returns the value the reference's branch for its region returns,
+// computed by the same operations in the same order; values of the regions a
+// face is not in are discarded by the selects (never multiplied by zero).
+// The physical-state safety (HLLD_PHYSICAL_SAFETY -> HLLE) is applied under the
+// reference's conditions, and the zero-normal-B solver (HLLD.cpp:137-205)
+// keeps its own branch.
+template <class T> __device__ U<T> hlld_uniform(P<T> L, P<T> R, const Phys& ph) {  // HLLD.cpp:38-133, :152-205
     T Bx = (L.Bx + R.Bx) / 2;
     L.Bx = Bx; R.Bx = Bx;
     T cl = c_fast(L, L.Bx, ph), cr = c_fast(R, R.Bx, ph);
     T SL = dmin(L.vx - cl, R.vx - cr);
-    if (SL >= T(0)) return pflux<true>(L, ph);
     T SR = dmax(L.vx + cl, R.vx + cr);
-    if (SR <= T(0)) return pflux<true>(R, ph);
-    T cutoff = dmax(dot3(L.Bx, L.By, L.Bz, L.Bx, L.By, L.Bz) + dot3(R.Bx, R.By, R.Bz, R.Bx, R.By, R.Bz), T(1e-100));
+    const bool supL = SL >= T(0), supR = !supL && SR <= T(0);
+    T cutoff = dmax(dot3(L.Bx, L.By, L.Bz, L.Bx, L.By, L.Bz) + dot3(R.Bx, R.By, R.Bz, R.Bx, R.By, R.Bz), hlld_floor<T>());
     const bool zeroB = Bx * Bx < T(1e-12) * cutoff;
     T pTL = L.p + dot3(L.Bx, L.By, L.Bz, L.Bx, L.By, L.Bz) * T(ph.k1_8pi);
     T pTR = R.p + dot3(R.Bx, R.By, R.Bz, R.Bx, R.By, R.Bz) * T(ph.k1_8pi);
@@ -384,26 +419,116 @@ template <class T> __device__ U<T> hlld(P<T> L, P<T> R, const Phys& ph) {  // HL
     usL.rho = xL / (SL - SM);
     usR.rho = xR / (SR - SM);
     T pT = (xR * pTL - xL * pTR + xL * xR * (R.vx - L.vx)) / (xR - xL);
-    U<T> FsL = uzero<T>(), FsR = uzero<T>();
+    P<T> wsL, wsR;
+    if (zeroB && !supL && !supR) {  // HLLD_zero_B (HLLD.cpp:152-205)
+        wsL = pzero<T>(); wsR = pzero<T>();
+        if (SM >= T(0)) {
+            outer_star_zero(usL, L, SL, SM, pT, ph);
+            wsL = star_prim(usL, pT, ph);
+            if (ph.hlld_safety && (!physical<true>(wsL) || !finite<true>(usL))) return hlle<true>(L, R, ph);
+        }
+        if (SM <= T(0)) {
+            outer_star_zero(usR, R, SR, SM, pT, ph);
+            wsR = star_prim(usR, pT, ph);
+            if (ph.hlld_safety && (!physical<true>(wsR) || !finite<true>(usR))) return hlle<true>(L, R, ph);
+        }
+        if (SM > T(0)) return star_flux(L, usL, SL, ph);
+        if (SM < T(0)) return star_flux(R, usR, SR, ph);
+        T sql = dsqrt(wsL.rho), sqr = dsqrt(wsR.rho);
+        T a = sql / (sql + sqr), b = sqr / (sql + sqr);
+        return uadd<true>(umul<true>(star_flux(L, usL, SL, ph), a), umul<true>(star_flux(R, usR, SR, ph), b));
+    }
+    T SsL = SM - dabs(Bx) / (T(ph.sq4pi) * dsqrt(usL.rho));
+    T SsR = SM + dabs(Bx) / (T(ph.sq4pi) * dsqrt(usR.rho));
+    outer_star(usL, L, xL, SL, SM, pT, Bx, ph);
+    wsL = star_prim(usL, pT, ph);
+    outer_star(usR, R, xR, SR, SM, pT, Bx, ph);
+    wsR = star_prim(usR, pT, ph);
+    // region of the face, in the reference's order of tests
+    const bool sub = !supL && !supR;
+    const bool starL = SsL >= T(0), starR = !starL && SsR <= T(0);
+    const bool dstar = sub && !starL && !starR;
+    const bool useL = supL || (sub && (starL || (!starR && SM >= T(0))));
+    bool bad = false;
+    if (ph.hlld_safety && sub) {
+        bad = (SsR >= T(0) && (!physical<true>(wsL) || !physical<true>(usL, ph))) ||
+              (SsL <= T(0) && (!physical<true>(wsR) || !physical<true>(usR, ph)));
+    }
+    // double-star state (used by dstar faces only)
+    const P<T> ws = psel(SM >= T(0), wsL, wsR);
+    T sqL = dsqrt(wsL.rho), sqR = dsqrt(wsR.rho);
+    T a = sqL / (sqL + sqR), b = sqR / (sqL + sqR);
+    T sbx = Bx > T(0) ? T(ph.sq4pi) : (Bx < T(0) ? -T(ph.sq4pi) : T(0));
+    T sum = sqL + sqR;
+    P<T> wss;
+    U<T> uss;
+    wss.rho = ws.rho;
+    uss.rho = wss.rho;
+    wss.vx = SM;
+    wss.vy = wsL.vy * a + wsR.vy * b + (usR.By - usL.By) * sbx * T(ph.k1_4pi) / sum;
+    wss.vz = wsL.vz * a + wsR.vz * b + (usR.Bz - usL.Bz) * sbx * T(ph.k1_4pi) / sum;
+    uss.mx = wss.vx * uss.rho; uss.my = wss.vy * uss.rho; uss.mz = wss.vz * uss.rho;
+    wss.Bx = Bx;
+    wss.By = usR.By * a + usL.By * b + (wsR.vy - wsL.vy) * sbx * sqL * sqR / sum;
+    wss.Bz = usR.Bz * a + usL.Bz * b + (wsR.vz - wsL.vz) * sbx * sqL * sqR / sum;
+    uss.Bx = wss.Bx; uss.By = wss.By; uss.Bz = wss.Bz;
+    T dE = sbx * (dot3(ws.vx, ws.vy, ws.vz, ws.Bx, ws.By, ws.Bz) - dot3(wss.vx, wss.vy, wss.vz, wss.Bx, wss.By, wss.Bz)) *
+           T(ph.k1_4pi) * dsqrt(wss.rho);
+    uss.E = SM >= T(0) ? usL.E - dE : usR.E + dE;
+    if (ph.hlld_safety && dstar) {
+        wss.p = pT - dot3(wss.Bx, wss.By, wss.Bz, wss.Bx, wss.By, wss.Bz) * T(ph.k1_8pi);
+        bad = bad || !physical<true>(wss) || !finite<true>(uss);
+    }
+    // the upwind side's fluxes
+    const P<T> K = psel(useL, L, R);
+    const U<T> usK = usel<true>(useL, usL, usR);
+    const T SK = useL ? SL : SR, SsK = useL ? SsL : SsR;
+    const U<T> FK = pflux<true>(K, ph);
+    U<T> FsK = uadd<true>(FK, umul<true>(usub<true>(usK, cons<true>(K, ph)), SK));
+    FsK.Bx = 0;
+    const U<T> FssK = uadd<true>(FsK, umul<true>(usub<true>(uss, usK), SsK));
+    U<T> F = usel<true>(supL || supR, FK, usel<true>(dstar, FssK, FsK));
+    if (bad) F = hlle<true>(L, R, ph);
+    return F;
+}
+
+// HLLD in the reference's control flow: one branch per wave region, the
+// outer-star fluxes formed where they are returned.
+template <class T> __device__ U<T> hlld_branch(P<T> L, P<T> R, const Phys& ph) {  // HLLD.cpp:38-133, :152-205
+    T Bx = (L.Bx + R.Bx) / 2;
+    L.Bx = Bx; R.Bx = Bx;
+    T cl = c_fast(L, L.Bx, ph), cr = c_fast(R, R.Bx, ph);
+    T SL = dmin(L.vx - cl, R.vx - cr);
+    if (SL >= T(0)) return pflux<true>(L, ph);
+    T SR = dmax(L.vx + cl, R.vx + cr);
+    if (SR <= T(0)) return pflux<true>(R, ph);
+    T cutoff = dmax(dot3(L.Bx, L.By, L.Bz, L.Bx, L.By, L.Bz) + dot3(R.Bx, R.By, R.Bz, R.Bx, R.By, R.Bz), hlld_floor<T>());
+    const bool zeroB = Bx * Bx < T(1e-12) * cutoff;
+    T pTL = L.p + dot3(L.Bx, L.By, L.Bz, L.Bx, L.By, L.Bz) * T(ph.k1_8pi);
+    T pTR = R.p + dot3(R.Bx, R.By, R.Bz, R.Bx, R.By, R.Bz) * T(ph.k1_8pi);
+    T xL = L.rho * (SL - L.vx), xR = R.rho * (SR - R.vx);
+    T SM = ((xR * R.vx - xL * L.vx) - (pTR - pTL)) / (xR - xL);
+    U<T> usL = uzero<T>(), usR = uzero<T>();
+    usL.rho = xL / (SL - SM);
+    usR.rho = xR / (SR - SM);
+    T pT = (xR * pTL - xL * pTR + xL * xR * (R.vx - L.vx)) / (xR - xL);
     P<T> wsL = pzero<T>(), wsR = pzero<T>();
     if (zeroB) {
         if (SM >= T(0)) {
             outer_star_zero(usL, L, SL, SM, pT, ph);
             wsL = star_prim(usL, pT, ph);
             if (ph.hlld_safety && (!physical<true>(wsL) || !finite<true>(usL))) return hlle<true>(L, R, ph);
-            FsL = uadd<true>(pflux<true>(L, ph), umul<true>(usub<true>(usL, cons<true>(L, ph)), SL)); FsL.Bx = 0;
         }
         if (SM <= T(0)) {
             outer_star_zero(usR, R, SR, SM, pT, ph);
             wsR = star_prim(usR, pT, ph);
             if (ph.hlld_safety && (!physical<true>(wsR) || !finite<true>(usR))) return hlle<true>(L, R, ph);
-            FsR = uadd<true>(pflux<true>(R, ph), umul<true>(usub<true>(usR, cons<true>(R, ph)), SR)); FsR.Bx = 0;
         }
-        if (SM > T(0)) return FsL;
-        if (SM < T(0)) return FsR;
+        if (SM > T(0)) return star_flux(L, usL, SL, ph);
+        if (SM < T(0)) return star_flux(R, usR, SR, ph);
         T sql = dsqrt(wsL.rho), sqr = dsqrt(wsR.rho);
         T a = sql / (sql + sqr), b = sqr / (sql + sqr);
-        return uadd<true>(umul<true>(FsL, a), umul<true>(FsR, b));
+        return uadd<true>(umul<true>(star_flux(L, usL, SL, ph), a), umul<true>(star_flux(R, usR, SR, ph), b));
     }
     T SsL = SM - dabs(Bx) / (T(ph.sq4pi) * dsqrt(usL.rho));
     T SsR = SM + dabs(Bx) / (T(ph.sq4pi) * dsqrt(usR.rho));
@@ -411,17 +536,15 @@ template <class T> __device__ U<T> hlld(P<T> L, P<T> R, const Phys& ph) {  // HL
         outer_star(usL, L, xL, SL, SM, pT, Bx, ph);
         wsL = star_prim(usL, pT, ph);
         if (ph.hlld_safety && (!physical<true>(wsL) || !physical<true>(usL, ph))) return hlle<true>(L, R, ph);
-        FsL = uadd<true>(pflux<true>(L, ph), umul<true>(usub<true>(usL, cons<true>(L, ph)), SL)); FsL.Bx = 0;
     }
     if (SsL <= T(0)) {
         outer_star(usR, R, xR, SR, SM, pT, Bx, ph);
         wsR = star_prim(usR, pT, ph);
         if (ph.hlld_safety && (!physical<true>(wsR) || !physical<true>(usR, ph))) return hlle<true>(L, R, ph);
-        FsR = uadd<true>(pflux<true>(R, ph), umul<true>(usub<true>(usR, cons<true>(R, ph)), SR)); FsR.Bx = 0;
     }
-    if (SsL >= T(0)) return FsL;
-    if (SsR <= T(0)) return FsR;
-    const P<T> ws = SM >= T(0) ? wsL : wsR;
+    if (SsL >= T(0)) return star_flux(L, usL, SL, ph);
+    if (SsR <= T(0)) return star_flux(R, usR, SR, ph);
+    const P<T> ws = psel(SM >= T(0), wsL, wsR);
     T sqL = dsqrt(wsL.rho), sqR = dsqrt(wsR.rho);
     T a = sqL / (sqL + sqR), b = sqR / (sqL + sqR);
     T sbx = Bx > T(0) ? T(ph.sq4pi) : (Bx < T(0) ? -T(ph.sq4pi) : T(0));
@@ -445,8 +568,17 @@ template <class T> __device__ U<T> hlld(P<T> L, P<T> R, const Phys& ph) {  // HL
         wss.p = pT - dot3(wss.Bx, wss.By, wss.Bz, wss.Bx, wss.By, wss.Bz) * T(ph.k1_8pi);
         if (!physical<true>(wss) || !finite<true>(uss)) return hlle<true>(L, R, ph);
     }
-    if (SM >= T(0)) return uadd<true>(FsL, umul<true>(usub<true>(uss, usL), SsL));
-    return uadd<true>(FsR, umul<true>(usub<true>(uss, usR), SsR));
+    if (SM >= T(0)) return uadd<true>(star_flux(L, usL, SL, ph), umul<true>(usub<true>(uss, usL), SsL));
+    return uadd<true>(star_flux(R, usR, SR, ph), umul<true>(usub<true>(uss, usR), SsR));
+}
+
+// FP32 (mixed form) runs the wavefront-uniform solver, FP64 the branching one:
+// measured on MI355X at 512^3, the uniform stream saves time in FP32 and
+// costs time in FP64 (both outer stars always built).  Both return the
+// same bits for every face.
+template <class T> __device__ __forceinline__ U<T> hlld(const P<T>& L, const P<T>& R, const Phys& ph) {
+    if constexpr (sizeof(T) == 4) return hlld_uniform<T>(L, R, ph);
+    else return hlld_branch<T>(L, R, ph);
 }
 
 // ---- exact solver (Exact.cpp), hydro fallback only

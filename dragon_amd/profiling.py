@@ -42,6 +42,7 @@ _CLASSES = [
     (r"k_init_B<", "init (initialize_B_fields)"),
     (r"k_divb<", "diagnostic (div B)"),
     (r"k_(aos_to_soa|soa_to_aos)", "layout transfer"),
+    (r"kb_(\w+)", lambda m: "batch " + m.group(1)),
 ]
 STEP_CLASSES_EXCLUDED = ("init (initialize_B_fields)", "diagnostic (div B)", "layout transfer")
 
