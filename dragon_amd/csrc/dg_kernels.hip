@@ -700,42 +700,143 @@ KB __global__ void k_emf2(FSoa FX, FSoa FY, Soa W, ERef ref, Ext ce, Ext fe, Box
     emf2_body<FROM_W>(FX, FY, W, ref, ce, fe, box, upwind, Ex, Ey, Ez, tid());
 }
 
-// 3D corner EMF (Electric.cpp:38-47, 74-100).
+// 3D corner EMF (Electric.cpp:38-47, 74-100) of the edge triple at (i,j,k).
+// fx/fy/fz(f, di, dj, dk) give flux component f (0 mass, 1 and 2 the two
+// transverse B components in x,y,z order) of the x/y/z flux at (i+di, j+dj,
+// k+dk), d in {-1, 0}; R(c, di, dj, dk) component c of the reference EMF of
+// that cell.
+template <class FA, class GA, class HA, class RA>
+__device__ __forceinline__ void emf3_eval(FA fx, GA fy, HA fz, RA R, int upwind, double& ex, double& ey, double& ez) {
+    // fx: {rho, By, Bz}; fy: {rho, Bx, Bz}; fz: {rho, Bx, By}
+    ex = (fz(2, 0, -1, 0) - fy(2, 0, 0, -1) + fz(2, 0, 0, 0) - fy(2, 0, 0, 0)) * 0.25;
+    ey = (fx(2, 0, 0, -1) - fz(1, -1, 0, 0) + fx(2, 0, 0, 0) - fz(1, 0, 0, 0)) * 0.25;
+    ez = (fy(1, -1, 0, 0) - fx(1, 0, -1, 0) + fy(1, 0, 0, 0) - fx(1, 0, 0, 0)) * 0.25;
+    if (upwind) {
+        ey += upwind_corr(fx(0, 0, 0, 0), -fz(1, -1, 0, 0), -fz(1, 0, 0, 0), R(1, -1, 0, 0), R(1, 0, 0, 0));
+        ey += upwind_corr(fx(0, 0, 0, -1), -fz(1, -1, 0, 0), -fz(1, 0, 0, 0), R(1, -1, 0, -1), R(1, 0, 0, -1));
+        ez += upwind_corr(fx(0, 0, 0, 0), fy(1, -1, 0, 0), fy(1, 0, 0, 0), R(2, -1, 0, 0), R(2, 0, 0, 0));
+        ez += upwind_corr(fx(0, 0, -1, 0), fy(1, -1, 0, 0), fy(1, 0, 0, 0), R(2, -1, -1, 0), R(2, 0, -1, 0));
+        ex += upwind_corr(fy(0, 0, 0, 0), fz(2, 0, -1, 0), fz(2, 0, 0, 0), R(0, 0, -1, 0), R(0, 0, 0, 0));
+        ex += upwind_corr(fy(0, 0, 0, -1), fz(2, 0, -1, 0), fz(2, 0, 0, 0), R(0, 0, -1, -1), R(0, 0, 0, -1));
+        ez += upwind_corr(fy(0, 0, 0, 0), -fx(1, 0, -1, 0), -fx(1, 0, 0, 0), R(2, 0, -1, 0), R(2, 0, 0, 0));
+        ez += upwind_corr(fy(0, -1, 0, 0), -fx(1, 0, -1, 0), -fx(1, 0, 0, 0), R(2, -1, -1, 0), R(2, -1, 0, 0));
+        ex += upwind_corr(fz(0, 0, 0, 0), -fy(2, 0, 0, -1), -fy(2, 0, 0, 0), R(0, 0, 0, -1), R(0, 0, 0, 0));
+        ex += upwind_corr(fz(0, 0, -1, 0), -fy(2, 0, 0, -1), -fy(2, 0, 0, 0), R(0, 0, -1, -1), R(0, 0, -1, 0));
+        ey += upwind_corr(fz(0, 0, 0, 0), fx(2, 0, 0, -1), fx(2, 0, 0, 0), R(1, 0, 0, -1), R(1, 0, 0, 0));
+        ey += upwind_corr(fz(0, -1, 0, 0), fx(2, 0, 0, -1), fx(2, 0, 0, 0), R(1, -1, 0, -1), R(1, -1, 0, 0));
+    }
+}
+// flux component slots of emf3_eval per axis: x {rho, By, Bz}, y {rho, Bx, Bz}, z {rho, Bx, By}
+__device__ __forceinline__ int emf_fld(int ax, int f) {
+    return f == 0 ? U_RHO : ax == 0 ? (f == 1 ? U_BY : U_BZ) : ax == 1 ? (f == 1 ? U_BX : U_BZ) : (f == 1 ? U_BX : U_BY);
+}
+
+// One thread per edge triple, every operand read from memory.
 template <bool FROM_W>
 KB __global__ void k_emf3(FSoa FX, FSoa FY, FSoa FZ, Soa W, ERef ref, Ext ce, Ext fe, Box box, int upwind, double* Ex,
                           double* Ey, double* Ez) {
     int i, j, k;
     if (!box.point(tid(), i, j, k)) return;
-    auto f = [&](const FSoa& F, int fld, int a, int b, int c) { return double(F(fld, ce.at(a, b, c))); };
-    double ex = (f(FZ, U_BY, i, j - 1, k) - f(FY, U_BZ, i, j, k - 1) + f(FZ, U_BY, i, j, k) - f(FY, U_BZ, i, j, k)) * 0.25;
-    double ey = (f(FX, U_BZ, i, j, k - 1) - f(FZ, U_BX, i - 1, j, k) + f(FX, U_BZ, i, j, k) - f(FZ, U_BX, i, j, k)) * 0.25;
-    double ez = (f(FY, U_BX, i - 1, j, k) - f(FX, U_BY, i, j - 1, k) + f(FY, U_BX, i, j, k) - f(FX, U_BY, i, j, k)) * 0.25;
-    if (upwind) {
-        auto R = [&](int comp, int a, int b, int c) -> double {
-            if (FROM_W) {
-                const long q = ce.at(a, b, c);
-                double x, y, z;
-                cross(W(F_BX, q), W(F_BY, q), W(F_BZ, q), W(F_VX, q), W(F_VY, q), W(F_VZ, q), x, y, z);
-                return comp == 0 ? x : comp == 1 ? y : z;
-            }
-            const long q = fe.at(a, b, c);
-            return comp == 0 ? ref.x[q] : comp == 1 ? ref.y[q] : ref.z[q];
-        };
-        ey += upwind_corr(f(FX, U_RHO, i, j, k), -f(FZ, U_BX, i - 1, j, k), -f(FZ, U_BX, i, j, k), R(1, i - 1, j, k), R(1, i, j, k));
-        ey += upwind_corr(f(FX, U_RHO, i, j, k - 1), -f(FZ, U_BX, i - 1, j, k), -f(FZ, U_BX, i, j, k), R(1, i - 1, j, k - 1), R(1, i, j, k - 1));
-        ez += upwind_corr(f(FX, U_RHO, i, j, k), f(FY, U_BX, i - 1, j, k), f(FY, U_BX, i, j, k), R(2, i - 1, j, k), R(2, i, j, k));
-        ez += upwind_corr(f(FX, U_RHO, i, j - 1, k), f(FY, U_BX, i - 1, j, k), f(FY, U_BX, i, j, k), R(2, i - 1, j - 1, k), R(2, i, j - 1, k));
-        ex += upwind_corr(f(FY, U_RHO, i, j, k), f(FZ, U_BY, i, j - 1, k), f(FZ, U_BY, i, j, k), R(0, i, j - 1, k), R(0, i, j, k));
-        ex += upwind_corr(f(FY, U_RHO, i, j, k - 1), f(FZ, U_BY, i, j - 1, k), f(FZ, U_BY, i, j, k), R(0, i, j - 1, k - 1), R(0, i, j, k - 1));
-        ez += upwind_corr(f(FY, U_RHO, i, j, k), -f(FX, U_BY, i, j - 1, k), -f(FX, U_BY, i, j, k), R(2, i, j - 1, k), R(2, i, j, k));
-        ez += upwind_corr(f(FY, U_RHO, i - 1, j, k), -f(FX, U_BY, i, j - 1, k), -f(FX, U_BY, i, j, k), R(2, i - 1, j - 1, k), R(2, i - 1, j, k));
-        ex += upwind_corr(f(FZ, U_RHO, i, j, k), -f(FY, U_BZ, i, j, k - 1), -f(FY, U_BZ, i, j, k), R(0, i, j, k - 1), R(0, i, j, k));
-        ex += upwind_corr(f(FZ, U_RHO, i, j - 1, k), -f(FY, U_BZ, i, j, k - 1), -f(FY, U_BZ, i, j, k), R(0, i, j - 1, k - 1), R(0, i, j - 1, k));
-        ey += upwind_corr(f(FZ, U_RHO, i, j, k), f(FX, U_BZ, i, j, k - 1), f(FX, U_BZ, i, j, k), R(1, i, j, k - 1), R(1, i, j, k));
-        ey += upwind_corr(f(FZ, U_RHO, i - 1, j, k), f(FX, U_BZ, i, j, k - 1), f(FX, U_BZ, i, j, k), R(1, i - 1, j, k - 1), R(1, i - 1, j, k));
-    }
+    auto fx = [&](int f, int a, int b, int c) { return double(FX(emf_fld(0, f), ce.at(i + a, j + b, k + c))); };
+    auto fy = [&](int f, int a, int b, int c) { return double(FY(emf_fld(1, f), ce.at(i + a, j + b, k + c))); };
+    auto fz = [&](int f, int a, int b, int c) { return double(FZ(emf_fld(2, f), ce.at(i + a, j + b, k + c))); };
+    auto R = [&](int comp, int a, int b, int c) -> double {
+        if (FROM_W) {
+            const long q = ce.at(i + a, j + b, k + c);
+            double x, y, z;
+            cross(W(F_BX, q), W(F_BY, q), W(F_BZ, q), W(F_VX, q), W(F_VY, q), W(F_VZ, q), x, y, z);
+            return comp == 0 ? x : comp == 1 ? y : z;
+        }
+        const long q = fe.at(i + a, j + b, k + c);
+        return comp == 0 ? ref.x[q] : comp == 1 ? ref.y[q] : ref.z[q];
+    };
+    double ex, ey, ez;
+    emf3_eval(fx, fy, fz, R, upwind, ex, ey, ez);
     const long e = fe.at(i, j, k);
     Ex[e] = ex; Ey[e] = ey; Ez[e] = ez;
+}
+
+// The same edges blocked for reuse: a workgroup owns a (j,k) tile of
+// EMF_TJ x EMF_TK edges and walks a run of i-planes.  Each cell's operands
+// (9 flux components and its reference EMF, computed once) are read from
+// memory once per plane into LDS, with a one-cell halo on the low j and k
+// sides; the previous plane stays in the other LDS buffer for the i-1
+// neighbours.  Every edge is evaluated by emf3_eval, as in k_emf3.
+constexpr int EMF_TK = 64, EMF_TJ = 4, EMF_CELLS = (EMF_TJ + 1) * (EMF_TK + 1);
+struct EmfPlane {
+    double r[3][EMF_CELLS];
+    FT f[3][3][EMF_CELLS];  // [axis][slot][cell]
+};
+template <bool FROM_W>
+KB __global__ void k_emf3_tiled(FSoa FX, FSoa FY, FSoa FZ, Soa W, ERef ref, Ext ce, Ext fe, Box box, int upwind, int iplanes,
+                                double* Ex, double* Ey, double* Ez) {
+    __shared__ EmfPlane pl[2];
+    const int nkt = (box.hi[2] - box.lo[2] + EMF_TK - 1) / EMF_TK;
+    const int njt = (box.hi[1] - box.lo[1] + EMF_TJ - 1) / EMF_TJ;
+    int b = blockIdx.x;
+    const int kt = b % nkt;
+    b /= nkt;
+    const int jt = b % njt;
+    const int ic = b / njt;
+    const int k0 = box.lo[2] + kt * EMF_TK, j0 = box.lo[1] + jt * EMF_TJ;
+    const int ib = box.lo[0] + ic * iplanes, ie = min(box.hi[0], ib + iplanes);
+    auto load = [&](EmfPlane& P, int i) {
+        for (int c = threadIdx.x; c < EMF_CELLS; c += blockDim.x) {
+            const int j = j0 + c / (EMF_TK + 1) - 1, k = k0 + c % (EMF_TK + 1) - 1;
+            if (j >= box.hi[1] || k >= box.hi[2]) continue;  // outside the box: no edge reads it
+            const long q = ce.at(i, j, k);
+            for (int f = 0; f < 3; f++) {
+                P.f[0][f][c] = FX(emf_fld(0, f), q);
+                P.f[1][f][c] = FY(emf_fld(1, f), q);
+                P.f[2][f][c] = FZ(emf_fld(2, f), q);
+            }
+            if (upwind) {
+                if (FROM_W) {
+                    cross(W(F_BX, q), W(F_BY, q), W(F_BZ, q), W(F_VX, q), W(F_VY, q), W(F_VZ, q), P.r[0][c], P.r[1][c], P.r[2][c]);
+                } else {
+                    const long e = fe.at(i, j, k);
+                    P.r[0][c] = ref.x[e]; P.r[1][c] = ref.y[e]; P.r[2][c] = ref.z[e];
+                }
+            }
+        }
+    };
+    const int tk = threadIdx.x % EMF_TK, tj = threadIdx.x / EMF_TK;
+    const int j = j0 + tj, k = k0 + tk;
+    const bool live = j < box.hi[1] && k < box.hi[2];
+    auto cell = [&](int dj, int dk) { return (tj + dj + 1) * (EMF_TK + 1) + (tk + dk + 1); };
+    int cur = 0;
+    load(pl[1], ib - 1);
+    for (int i = ib; i < ie; i++) {
+        load(pl[cur], i);
+        __syncthreads();
+        const EmfPlane& C = pl[cur];
+        const EmfPlane& Pv = pl[1 - cur];
+        auto pick = [&](int a) -> const EmfPlane& { return a < 0 ? Pv : C; };
+        auto fx = [&](int f, int a, int b2, int c2) { return double(pick(a).f[0][f][cell(b2, c2)]); };
+        auto fy = [&](int f, int a, int b2, int c2) { return double(pick(a).f[1][f][cell(b2, c2)]); };
+        auto fz = [&](int f, int a, int b2, int c2) { return double(pick(a).f[2][f][cell(b2, c2)]); };
+        auto R = [&](int comp, int a, int b2, int c2) { return pick(a).r[comp][cell(b2, c2)]; };
+        if (live) {
+            double ex, ey, ez;
+            emf3_eval(fx, fy, fz, R, upwind, ex, ey, ez);
+            const long e = fe.at(i, j, k);
+            Ex[e] = ex; Ey[e] = ey; Ez[e] = ez;
+        }
+        __syncthreads();  // the previous plane's buffer is refilled next
+        cur = 1 - cur;
+    }
+}
+
+// i-planes per workgroup of the tiled EMF (one extra plane is read per run)
+constexpr int EMF_IPLANES = 32;
+template <bool FROM_W>
+static void emf3_tiled(const FSoa& FX, const FSoa& FY, const FSoa& FZ, const Soa& W, const ERef& ref, const Ext& ce, const Ext& fe,
+                       const Box& eb, int upwind, double* Ex, double* Ey, double* Ez, hipStream_t s) {
+    const long nkt = (eb.hi[2] - eb.lo[2] + EMF_TK - 1) / EMF_TK, njt = (eb.hi[1] - eb.lo[1] + EMF_TJ - 1) / EMF_TJ;
+    const long nic = (eb.hi[0] - eb.lo[0] + EMF_IPLANES - 1) / EMF_IPLANES;
+    if (nkt <= 0 || njt <= 0 || nic <= 0) return;
+    hipLaunchKernelGGL((k_emf3_tiled<FROM_W>), dim3((unsigned)(nkt * njt * nic)), dim3(256), 0, s, FX, FY, FZ, W, ref, ce, fe, eb, upwind,
+                       EMF_IPLANES, Ex, Ey, Ez);
 }
 
 // Faraday (B_fields.cpp:21-66) fused with the clone: Bout = Bin everywhere,
@@ -1171,8 +1272,7 @@ static void step_fused(const StepCtx& c, hipStream_t s) {
                 hipLaunchKernelGGL((k_emf2<true>), dim3(blocks(eb.count())), dim3(256), 0, s, fs(c.Fp[0]), fs(c.Fp[1]), c.W, none, ce, c.fe, eb, 1,
                                    c.E0[0], c.E0[1], c.E0[2]);
             else
-                hipLaunchKernelGGL((k_emf3<true>), dim3(blocks(eb.count())), dim3(256), 0, s, fs(c.Fp[0]), fs(c.Fp[1]), fs(c.Fp[2]), c.W, none, ce,
-                                   c.fe, eb, 1, c.E0[0], c.E0[1], c.E0[2]);
+                emf3_tiled<true>(fs(c.Fp[0]), fs(c.Fp[1]), fs(c.Fp[2]), c.W, none, ce, c.fe, eb, 1, c.E0[0], c.E0[1], c.E0[2], s);
             mark(c, K_EMF_PRED);
             hipLaunchKernelGGL((k_faraday<D>), dim3(blocks(c.fe.size)), dim3(256), 0, s, c.E0[0], c.E0[1], c.E0[2], c.B[0], c.B[1],
                                c.B[2], c.Bh[0], c.Bh[1], c.Bh[2], c.fe, 1, 0.5 * dt_d[0], 0.5 * dt_d[1], 0.5 * dt_d[2]);
@@ -1196,9 +1296,11 @@ static void step_fused(const StepCtx& c, hipStream_t s) {
         if constexpr (D == 2)
             hipLaunchKernelGGL((k_emf2<false>), dim3(blocks(eb.count())), dim3(256), 0, s, fs(c.F[0]), fs(c.F[1]), c.W, eh, ce, c.fe, eb, c.ctu,
                                c.E0[0], c.E0[1], c.E0[2]);
+        else if constexpr (sizeof(FT) == 8)  // FP64 fluxes: the LDS planes allow 2 workgroups per CU, slower than k_emf3 here
+            hipLaunchKernelGGL((k_emf3<false>), dim3(blocks(eb.count())), dim3(256), 0, s, fs(c.F[0]), fs(c.F[1]), fs(c.F[2]), c.W, eh, ce, c.fe,
+                               eb, c.ctu, c.E0[0], c.E0[1], c.E0[2]);
         else
-            hipLaunchKernelGGL((k_emf3<false>), dim3(blocks(eb.count())), dim3(256), 0, s, fs(c.F[0]), fs(c.F[1]), fs(c.F[2]), c.W, eh, ce, c.fe, eb,
-                               c.ctu, c.E0[0], c.E0[1], c.E0[2]);
+            emf3_tiled<false>(fs(c.F[0]), fs(c.F[1]), fs(c.F[2]), c.W, eh, ce, c.fe, eb, c.ctu, c.E0[0], c.E0[1], c.E0[2], s);
         mark(c, K_EMF);
         hipLaunchKernelGGL((k_faraday<D>), dim3(blocks(c.fe.size)), dim3(256), 0, s, c.E0[0], c.E0[1], c.E0[2], c.B[0], c.B[1], c.B[2],
                            c.Bn[0], c.Bn[1], c.Bn[2], c.fe, 0, dt_d[0], dt_d[1], dt_d[2]);

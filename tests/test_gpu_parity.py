@@ -69,3 +69,20 @@ def test_hydro_3d_ctu12():
 def test_fp64_parity_100_steps_2d_mhd():
     """<= 1e-10 relative L1 per variable after 100 steps."""
     run_pair(P.orszag_tang_2d(48), steps=100)
+
+
+def test_mhd_3d_rectangular():
+    """Non-cubic 3D MHD (CTU + HLLD + CT, van Leer) on 40 x 9 x 70 cells with a
+    random state: several i-runs, j-tiles and k-tiles of the blocked EMF pass,
+    partial tiles on every side."""
+    shape = (40, 9, 70)
+    g = P.ghosts_for(True)
+    w, B = P.empty_state(shape, g, True)
+    rng = np.random.default_rng(7)
+    wi = P.interior(w, g, shape)
+    wi[..., 0] = 1.0 + 0.3 * rng.random(shape)
+    wi[..., 1] = 1.0 + 0.3 * rng.random(shape)
+    wi[..., 2:5] = 0.5 * (rng.random(shape + (3,)) - 0.5)
+    P.interior(B, g, shape, extra=1)[..., 0] = 0.2
+    cfg = dict(mhd=1, gamma=5.0 / 3.0, solver=5, limiter=2, ctu=1, ct_energy=2)
+    run_pair((shape, (1 / 40, 1 / 9, 1 / 70), cfg, [(1, 63)], w, B, {"init_B": True}), steps=5)
