@@ -100,6 +100,8 @@ def lib():
             "dg_cfl_dt": (C.c_int, [vp, _dp, C.POINTER(_Status)]),
             "dg_step": (C.c_int, [vp, C.c_double, C.POINTER(_Status)]),
             "dg_advance": (C.c_int, [vp, C.c_double, C.c_int, C.c_long, C.POINTER(_Stats), C.POINTER(_Status)]),
+            "dg_split_step": (C.c_int, [vp, C.c_double, C.POINTER(_Status)]),
+            "dg_advance_split": (C.c_int, [vp, C.c_double, C.c_int, C.c_long, C.POINTER(_Stats), C.POINTER(_Status)]),
             "dg_init_B": (C.c_int, [vp]),
             "dg_last_error": (C.c_char_p, [vp]),
             "dg_stream": (vp, [vp]),
@@ -151,6 +153,9 @@ class Config:
     hlld_safety: int = 1
     timestep_tol: float = 1e-14
     precision: int = FP64
+    # DIMENSION_UNSPLIT: Grid.advance runs advance_unsplit when set, advance_split
+    # (Strang splitting, hydro only) when not (CFL.cpp:151-157)
+    dimension_unsplit: int = 1
 
 
 # ---------------------------------------------------------------- boundary
@@ -408,8 +413,25 @@ class Grid:
         self.t_advanced = float(S.t_advanced)  # simulated time of this call
         return int(S.steps), int(S.failures)
 
+    def split_step(self, dt: float):
+        """Grid2D/3D::split_step (Splitting.cpp:118-216), hydro; raises
+        DragonError and commits nothing on failure."""
+        st = _Status()
+        self._chk(lib().dg_split_step(self._h, float(dt), C.byref(st)), st)
+
+    def advance_split(self, dt: float, check_cfl=True, max_steps=-1):
+        """Grid::advance_split (CFL.cpp:160-184); returns (steps, failed attempts)."""
+        self._push_bc()
+        st, S = _Status(), _Stats()
+        self._chk(lib().dg_advance_split(self._h, float(dt), int(check_cfl), int(max_steps), C.byref(S), C.byref(st)), st)
+        self.t_advanced = float(S.t_advanced)
+        return int(S.steps), int(S.failures)
+
     def advance(self, dt: float, check_cfl=True, max_steps=-1):
-        return self.advance_unsplit(dt, check_cfl, max_steps)
+        """Grid::advance (CFL.cpp:151-157): unsplit or split by Config.dimension_unsplit."""
+        if self.config.dimension_unsplit:
+            return self.advance_unsplit(dt, check_cfl, max_steps)
+        return self.advance_split(dt, check_cfl, max_steps)
 
     def initialize_B_fields(self):
         self._push_bc()

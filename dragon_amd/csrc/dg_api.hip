@@ -33,6 +33,8 @@ struct dg_grid {
     double *HS = nullptr, *F = nullptr, *Fp = nullptr, *T = nullptr;
     bool fused = true;
     int march_chunk = 0;
+    double* Ws = nullptr;       // split step: third state buffer (allocated on first use)
+    int sweep_step = 0;         // split step: rotation of the sweep order (Grid.hpp:93, 135)
     double *E0 = nullptr, *Eh = nullptr;
     double* cflpart = nullptr;  // per-block CFL maxima of the committed state, written by the apply pass
     long ncflpart = 0;
@@ -221,7 +223,7 @@ int dg_destroy(dg_grid* g) {
     if (!g) return DG_OK;
     int rc = DG_OK;
     if (g->s && hipStreamSynchronize(g->s) != hipSuccess) rc = DG_ERR_HIP;
-    double* bufs[] = {g->W, g->Wn, g->B, g->Bn, g->Bh, g->HS, g->F, g->Fp, g->T, g->E0, g->Eh, g->cflpart};
+    double* bufs[] = {g->W, g->Wn, g->B, g->Bn, g->Bh, g->HS, g->F, g->Fp, g->T, g->E0, g->Eh, g->cflpart, g->Ws};
     for (double* p : bufs)
         if (p && hipFree(p) != hipSuccess) rc = DG_ERR_HIP;
     if (g->dev_words && hipFree(g->dev_words) != hipSuccess) rc = DG_ERR_HIP;
@@ -453,8 +455,78 @@ int dg_step(dg_grid* g, double dt, dg_status* st) {
     return DG_OK;
 }
 
-int dg_advance(dg_grid* g, double dt, int check_cfl, long max_steps, dg_stats* stats, dg_status* st) {
+// Grid2D/3D::split_step (Splitting.cpp:118-216): Strang splitting of hydro,
+// the sweep order rotating with sweep_step (2 orders in 2D, 6 in 3D).  The
+// sweeps run W -> Wn -> Ws -> Wn [-> Ws -> Wn], each after the boundary pass
+// on its input; W is kept, so a failed step commits nothing.
+int dg_split_step(dg_grid* g, double dt, dg_status* st) {
     if (!g) return DG_ERR_ARG;
+    const int D = g->d.dim;
+    if (D == 1) return dg_step(g, dt, st);  // Grid1D::split_step is unsplit_step (Splitting.cpp:98)
+    if (g->d.mhd) {
+        const char* msg = "MHD requires Unsplit dimensions. Please enable DIMENSION_UNSPLIT in Config.h";  // Electric.cpp:17-18
+        set_status(st, DG_ERR_UNSUPPORTED, 0, 0, 0, msg);
+        return fail(g, DG_ERR_UNSUPPORTED, msg);
+    }
+    if (!g->Ws && hipMalloc((void**)&g->Ws, sizeof(double) * g->nf * g->ce.size) != hipSuccess) return fail(g, DG_ERR_HIP, "hipMalloc");
+    g->cfl_cached = false;
+    launch_fill_u64(g->dev_words, NO_ERR, 1, g->s);
+    static const int order3[6][3] = {{0, 1, 2}, {2, 0, 1}, {1, 2, 0}, {2, 1, 0}, {0, 2, 1}, {1, 0, 2}};  // Splitting.cpp:160-201
+    int ax[5], n = 0;
+    double h[5];
+    if (D == 2) {
+        const int a = g->sweep_step % 2 == 0 ? 0 : 1, b = 1 - a;
+        ax[0] = a; ax[1] = b; ax[2] = a;
+        h[0] = dt / 2; h[1] = dt; h[2] = dt / 2;
+        n = 3;
+    } else {
+        const int* o = order3[g->sweep_step % 6];
+        ax[0] = o[0]; ax[1] = o[1]; ax[2] = o[2]; ax[3] = o[1]; ax[4] = o[0];
+        h[0] = dt / 2; h[1] = dt / 2; h[2] = dt; h[3] = dt / 2; h[4] = dt / 2;
+        n = 5;
+    }
+    if (g->timing) HIPCHK(hipEventRecord(g->ev0, g->s));
+    double* in = g->W;
+    double* const W0 = g->W;
+    for (int q = 0; q < n; q++) {
+        double* out = q % 2 == 0 ? g->Wn : g->Ws;
+        g->W = in;  // the boundary pass runs on this sweep's input
+        int rc = dg_apply_boundary(g);
+        g->W = W0;
+        if (rc) return rc;
+        StepCtx c = make_ctx(g, h[q]);
+        tm_record(g, -1);
+        if (c.mixed) p32::launch_split_axis(c, ax[q], soa(in, g->ce.size), soa(out, g->ce.size), g->s);
+        else p64::launch_split_axis(c, ax[q], soa(in, g->ce.size), soa(out, g->ce.size), g->s);
+        tm_record(g, K_FLUX_X + ax[q]);
+        in = out;
+    }
+    HIPCHK(hipGetLastError());
+    if (g->timing) HIPCHK(hipEventRecord(g->ev1, g->s));
+    HIPCHK(hipMemcpyAsync(g->host_words, g->dev_words, sizeof(unsigned long long), hipMemcpyDeviceToHost, g->s));
+    HIPCHK(hipStreamSynchronize(g->s));
+    if (g->timing) {
+        float ms = 0;
+        HIPCHK(hipEventElapsedTime(&ms, g->ev0, g->ev1));
+        g->step_ms += ms;
+        g->timed_steps++;
+        tm_flush(g);
+    }
+    const unsigned long long key = g->host_words[0];
+    if (key != NO_ERR) {
+        int code;
+        std::string msg;
+        decode(g, key, st, &code, &msg);
+        return fail(g, code, msg);
+    }
+    std::swap(g->W, g->Wn);  // the last sweep wrote Wn (an odd number of sweeps)
+    g->sweep_step++;
+    set_status(st, DG_OK, 0, 0, 0, "");
+    return DG_OK;
+}
+
+// Grid::advance_unsplit / advance_split (CFL.cpp:160-238)
+static int advance_loop(dg_grid* g, double dt, int check_cfl, long max_steps, dg_stats* stats, dg_status* st, bool split) {
     dg_stats local{};
     dg_stats& S = stats ? *stats : local;
     S.steps = 0; S.failures = 0; S.t_advanced = 0;
@@ -471,9 +543,9 @@ int dg_advance(dg_grid* g, double dt, int check_cfl, long max_steps, dg_stats* s
             t1 = (c < dt) ? c : dt;  // std::min(dt, cfl)
         }
         while (true) {
-            rc = dg_step(g, t1, st);
+            rc = split ? dg_split_step(g, t1, st) : dg_step(g, t1, st);
             if (rc == DG_OK) break;
-            if (rc == DG_ERR_HIP || rc == DG_ERR_ARG) return rc;
+            if (rc == DG_ERR_HIP || rc == DG_ERR_ARG || rc == DG_ERR_UNSUPPORTED) return rc;
             S.failures++;
             t1 *= 0.5;
             if (t1 < tol) {
@@ -488,6 +560,16 @@ int dg_advance(dg_grid* g, double dt, int check_cfl, long max_steps, dg_stats* s
     }
     set_status(st, DG_OK, 0, 0, 0, "");
     return DG_OK;
+}
+
+int dg_advance(dg_grid* g, double dt, int check_cfl, long max_steps, dg_stats* stats, dg_status* st) {
+    if (!g) return DG_ERR_ARG;
+    return advance_loop(g, dt, check_cfl, max_steps, stats, st, false);
+}
+
+int dg_advance_split(dg_grid* g, double dt, int check_cfl, long max_steps, dg_stats* stats, dg_status* st) {
+    if (!g) return DG_ERR_ARG;
+    return advance_loop(g, dt, check_cfl, max_steps, stats, st, true);
 }
 
 int dg_init_B(dg_grid* g) {

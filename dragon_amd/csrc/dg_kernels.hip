@@ -1709,6 +1709,70 @@ void launch_batch_step(const BatchArgs& a, const BlockTable* bt, bool init_only,
     hipLaunchKernelGGL(kb_commit, dim3(pb), T, 0, s, a);
 }
 
+// ====================================================== Strang split step
+// One directional sweep of Grid2D/3D::split_step (advanceX/Y/Z,
+// Splitting.cpp:219-299): Godunov::sweep (Splitting.cpp:67-95) along every
+// line of axis AX, ghost lines included.  The faces come from the fused sweep
+// body over faces [-g+1, n+g-1] (first order at both ends, as the 1D sweep);
+// the update runs in the line's own frame (XY / XZ swap), where the
+// reference's y and z sweeps do their prim -> cons -> prim arithmetic, and
+// cell n+g-2 takes the sweep's trailing zero update.  Hydro only.
+template <class T> __device__ __forceinline__ P<T> to_line(const P<T>& w, int ax) { return ax == 0 ? w : ax == 1 ? swapXY(w) : swapXZ(w); }
+template <class T> __device__ __forceinline__ U<T> to_line(const U<T>& u, int ax) { return ax == 0 ? u : ax == 1 ? swapXY(u) : swapXZ(u); }
+template <int D, int AX>
+KB __global__ void k_apply_axis(Soa W, Soa Wn, FSoa F, Ext ce, double dt_d, Phys ph) {
+    const long m = tid();
+    if (m >= ce.size) return;
+    const int g = ce.g, n = ce.n[AX];
+    long r = m;
+    int idx[3] = {0, 0, 0};
+    const int Ny = D >= 2 ? ce.n[1] + 2 * g : 1, Nz = D >= 3 ? ce.n[2] + 2 * g : 1;
+    if (D == 3) { idx[2] = (int)(r % Nz) - g; r /= Nz; }
+    if (D >= 2) { idx[1] = (int)(r % Ny) - g; r /= Ny; }
+    idx[0] = (int)r - g;
+    const int a = idx[AX];
+    if (a < -g + 1 || a > n + g - 2) {
+        for (int f = 0; f < 5; f++) Wn(f, m) = W(f, m);
+        return;
+    }
+    P<double> w = to_line(load_p<double>(W, m, false), AX);
+    const U<double> fl = to_line(load_u(F, m, false), AX), fr = to_line(load_u(F, m + ce.s[AX], false), AX);
+    U<double> u = uadd<false>(cons<false>(w, ph), umul<false>(usub<false>(fl, fr), dt_d));
+    w = prim<false>(u, ph);
+    if (a == n + g - 2) {  // the sweep's last update: the same flux on both sides
+        u = uadd<false>(cons<false>(w, ph), umul<false>(usub<false>(fr, fr), dt_d));
+        w = prim<false>(u, ph);
+    }
+    store_p<double>(Wn, m, to_line(w, AX), false);
+}
+template <int D, int AX>
+static void split_axis_t(const StepCtx& c, const Soa& in, const Soa& out, hipStream_t s) {
+    if constexpr (AX < D) {
+        StepCtx cs = c;
+        cs.W = in;
+        const Ext& ce = c.ce;
+        const int g = ce.g;
+        int lo[3] = {0, 0, 0}, hi[3] = {1, 1, 1};
+        for (int d = 0; d < D; d++) {
+            lo[d] = d == AX ? -g + 1 : -g;
+            hi[d] = ce.n[d] + g;
+        }
+        const Box b = make_box(D, lo[0], hi[0], lo[1], hi[1], lo[2], hi[2]);
+        sweep<D, false, RT, false>(cs, AX, b, fs(c.F[AX]), ST_FLUX_X + AX, s);
+        hipLaunchKernelGGL((k_apply_axis<D, AX>), dim3(blocks(ce.size)), dim3(256), 0, s, in, out, fs(c.F[AX]), ce, c.dt / c.d[AX], c.ph);
+    }
+}
+void launch_split_axis(const StepCtx& c, int ax, Soa in, Soa out, hipStream_t s) {
+    if (c.dim == 2) {
+        if (ax == 0) split_axis_t<2, 0>(c, in, out, s);
+        else split_axis_t<2, 1>(c, in, out, s);
+    } else if (c.dim == 3) {
+        if (ax == 0) split_axis_t<3, 0>(c, in, out, s);
+        else if (ax == 1) split_axis_t<3, 1>(c, in, out, s);
+        else split_axis_t<3, 2>(c, in, out, s);
+    }
+}
+
 void launch_step(const StepCtx& c, hipStream_t s) {
     if (c.dim == 1) c.mhd ? step_t<1, true, RT>(c, s) : step_t<1, false, RT>(c, s);
     else if (c.dim == 2) c.mhd ? step_t<2, true, RT>(c, s) : step_t<2, false, RT>(c, s);

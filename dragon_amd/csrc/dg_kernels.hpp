@@ -125,6 +125,7 @@ struct BatchArgs {
 #define DG_LAUNCH_INTERFACE                                                                                     \
     void launch_batch_step(const BatchArgs& a, const BlockTable* bt, bool init_only, hipStream_t s);           \
     void launch_step(const StepCtx& c, hipStream_t s);                                                          \
+    void launch_split_axis(const StepCtx& c, int ax, Soa in, Soa out, hipStream_t s);  /* one split sweep (hydro) */ \
     void launch_bc(const BCArgs& a, bool faces, hipStream_t s);                                                 \
     void launch_cfl(const StepCtx& c, unsigned long long* out, hipStream_t s);                                  \
     void launch_cfl_tail(const StepCtx& c, long nparts, unsigned long long* out, hipStream_t s);                \

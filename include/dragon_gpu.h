@@ -123,6 +123,14 @@ int dg_cfl_dt(dg_grid* g, double* dt, dg_status* st);
 int dg_step(dg_grid* g, double dt, dg_status* st);
 /* Grid::advance_unsplit(dt, check_cfl) (CFL.cpp:213-238); max_steps < 0: no limit. */
 int dg_advance(dg_grid* g, double dt, int check_cfl, long max_steps, dg_stats* stats, dg_status* st);
+/* Grid2D/3D::split_step(dt) (Hydro/Godunov/Splitting.cpp:118-216): Strang-split
+ * directional Godunov sweeps (advanceX/Y/Z, :219-299), the order rotating
+ * per accepted step; hydro only (MHD returns DG_ERR_UNSUPPORTED, as
+ * MHD/Electric.cpp:17-18 refuses DIMENSION_UNSPLIT off).  1D: dg_step.  No
+ * commit on failure.  Grid::advance_split(dt, check_cfl) (Hydro/CFL/CFL.cpp:160-184)
+ * is dg_advance_split, the dg_advance loop around dg_split_step. */
+int dg_split_step(dg_grid* g, double dt, dg_status* st);
+int dg_advance_split(dg_grid* g, double dt, int check_cfl, long max_steps, dg_stats* stats, dg_status* st);
 /* Grid2D/3D::initialize_B_fields() (MHD/B_fields.cpp:117-149) */
 int dg_init_B(dg_grid* g);
 /* Error text of the last failing call on this handle. */

@@ -41,6 +41,7 @@ struct Config {
     int hlld_safety = 1;
     double timestep_tol = 1e-14;
     int precision = DG_FP64;
+    int dimension_unsplit = 1;  // DIMENSION_UNSPLIT: advance() runs advance_unsplit, else advance_split (hydro)
 };
 
 // PrimitiveState record (FluidElement.hpp:34-43).
@@ -156,6 +157,7 @@ public:
         w_.assign(dg_cell_count(h) * 8, 0.0);
         if (has_B_) B_.assign(dg_face_count(h) * 3, 0.0);
         tol_ = c.timestep_tol;
+        unsplit_ = c.dimension_unsplit != 0;
     }
     virtual ~GpuGrid() = default;
 
@@ -197,8 +199,20 @@ public:
     }
     // Grid::on_step_fail: true = retry with half the step (the reference's default)
     virtual bool on_step_fail(const std::exception&) { return true; }
+    // Grid2D/3D::split_step(dt) (Splitting.cpp:118-216): Strang-split sweeps, hydro only
+    virtual void split_step(double dt) {
+        dg_status st{};
+        check(dg_split_step(h_.get(), dt, &st), &st);
+    }
     // Grid::advance_unsplit(dt, check_cfl) (CFL.cpp:213-238); returns accepted steps
-    long advance_unsplit(double dt, bool check_cfl = true) {
+    long advance_unsplit(double dt, bool check_cfl = true) { return advance_loop(dt, check_cfl, false); }
+    // Grid::advance_split(dt, check_cfl) (CFL.cpp:160-184)
+    long advance_split(double dt, bool check_cfl = true) { return advance_loop(dt, check_cfl, true); }
+    // Grid::advance (CFL.cpp:151-157)
+    long advance(double dt, bool check_cfl = true) { return unsplit_ ? advance_unsplit(dt, check_cfl) : advance_split(dt, check_cfl); }
+
+protected:
+    long advance_loop(double dt, bool check_cfl, bool split) {
         push_bc();
         long steps = 0;
         while (dt > tol_) {
@@ -210,10 +224,11 @@ public:
             }
             while (true) {
                 try {
-                    unsplit_step(t1);
+                    if (split) split_step(t1);
+                    else unsplit_step(t1);
                     break;
                 } catch (const GpuError& e) {
-                    if (e.code == DG_ERR_HIP || e.code == DG_ERR_ARG || !on_step_fail(e)) throw;
+                    if (e.code == DG_ERR_HIP || e.code == DG_ERR_ARG || e.code == DG_ERR_UNSUPPORTED || !on_step_fail(e)) throw;
                     t1 *= 0.5;
                     if (t1 < tol_) throw GpuError(DG_ERR_DT_UNDERFLOW, "Timestep has fallen below minimum. Exiting");
                 }
@@ -223,9 +238,6 @@ public:
         }
         return steps;
     }
-    long advance(double dt, bool check_cfl = true) { return advance_unsplit(dt, check_cfl); }
-
-protected:
     long lin(int i, int j, int k, int extra) const {
         const long ex = n_[0] + extra + 2L * g_;
         const long ey = dim_ >= 2 ? n_[1] + extra + 2L * g_ : 1, ez = dim_ >= 3 ? n_[2] + extra + 2L * g_ : 1;
@@ -247,6 +259,7 @@ protected:
     int dim_ = 1, g_ = 0, n_[3] = {1, 1, 1};
     bool has_B_ = false;
     double tol_ = 1e-14;
+    bool unsplit_ = true;
     std::vector<double> w_, B_;
 };
 

@@ -88,6 +88,8 @@ def _load():
         "oc_grid_cfl": (C.c_int, [vp, _dp, C.c_char_p, C.c_int]),
         "oc_grid_step": (C.c_int, [vp, C.c_double, C.c_char_p, C.c_int]),
         "oc_grid_advance": (C.c_long, [vp, C.c_double, C.c_int, C.c_long, C.POINTER(C.c_long), C.c_char_p, C.c_int]),
+        "oc_grid_split_step": (C.c_int, [vp, C.c_double, C.c_char_p, C.c_int]),
+        "oc_grid_advance_split": (C.c_long, [vp, C.c_double, C.c_int, C.c_long, C.POINTER(C.c_long), C.c_char_p, C.c_int]),
         "oc_grid_faraday_init": (None, [vp, _dp]),
         "oc_riemann_flux": (C.c_int, [cp, _dp, _dp, C.c_double, C.c_int, _dp, C.c_char_p, C.c_int]),
         "oc_riemann_solver": (None, [cp, _dp, _dp, C.c_int, C.c_double, C.c_double, _dp]),
@@ -222,6 +224,20 @@ class Grid:
         err = C.create_string_buffer(256)
         fails = C.c_long(0)
         n = lib().oc_grid_advance(self.h, dt, int(check_cfl), int(max_steps), C.byref(fails), err, 256)
+        if n < 0:
+            raise OracleError(err.value.decode())
+        return int(n), int(fails.value)
+
+    def split_step(self, dt: float):
+        err = C.create_string_buffer(256)
+        if lib().oc_grid_split_step(self.h, dt, err, 256):
+            raise OracleError(err.value.decode())
+
+    def advance_split(self, dt: float, check_cfl=True, max_steps=-1):
+        """Grid::advance_split (Strang splitting, hydro); returns (accepted steps, failed attempts)."""
+        err = C.create_string_buffer(256)
+        fails = C.c_long(0)
+        n = lib().oc_grid_advance_split(self.h, dt, int(check_cfl), int(max_steps), C.byref(fails), err, 256)
         if n < 0:
             raise OracleError(err.value.decode())
         return int(n), int(fails.value)

@@ -97,6 +97,17 @@ long oc_grid_advance(void* h, double dt, int check_cfl, long max_steps, long* fa
         return ((Grid*)h)->advance(dt, check_cfl != 0, failures, max_steps);
     } catch (const std::exception& e) { put_err(err, n, e.what()); return -1; }
 }
+int oc_grid_split_step(void* h, double dt, char* err, int n) {
+    try {
+        ((Grid*)h)->split_step(dt);
+        return 0;
+    } catch (const std::exception& e) { put_err(err, n, e.what()); return 1; }
+}
+long oc_grid_advance_split(void* h, double dt, int check_cfl, long max_steps, long* failures, char* err, int n) {
+    try {
+        return ((Grid*)h)->advance(dt, check_cfl != 0, failures, max_steps, true);
+    } catch (const std::exception& e) { put_err(err, n, e.what()); return -1; }
+}
 // Faraday B = curl(A) initialisation (main.cpp:35-43, 57-67); A has no ghosts,
 // size (nx+1)(ny+1)[(nz+1)], 3 doubles per corner.
 void oc_grid_faraday_init(void* h, const double* A3) {

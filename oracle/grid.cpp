@@ -1042,8 +1042,59 @@ void Grid::unsplit_step(double dt) {
     else step3D(dt);
 }
 
-// Grid::advance_unsplit (CFL.cpp:213-238)
-long Grid::advance(double dt, bool check_cfl, long* failures, long max_steps) {
+// Grid2D/3D::advanceX/Y/Z (Splitting.cpp:219-299): the boundary pass, then
+// Godunov::sweep along every line of the axis, ghost lines included; the y and
+// z lines are swept in their own frame (XY / XZ swap in and out).
+void Grid::advance_axis(int ax, double dt) {
+    apply_boundary();
+    const int n = ax == 0 ? nx : ax == 1 ? ny : nz;
+    const double d = ax == 0 ? dx : ax == 1 ? dy : dz;
+    const int t0n = ax == 0 ? ny : nx;                        // first transverse dim (parallel loop)
+    const int t1n = dim == 3 ? (ax == 2 ? ny : nz) : 1;       // second transverse dim (3D only)
+    const int t1g = dim == 3 ? g : 0;
+    auto at = [&](int a, int p, int q) -> Prim& {             // a along ax, (p, q) transverse
+        if (dim == 2) return ax == 0 ? w(a, p) : w(p, a);
+        if (ax == 0) return w(a, p, q);
+        if (ax == 1) return w(p, a, q);
+        return w(p, q, a);
+    };
+    auto in = [&](const Prim& x) { return ax == 0 ? x : ax == 1 ? swapXY(x) : swapXZ(x); };
+    pfor(-g, t0n + g, [&](int p) {
+        Arr<Prim> line(n, g);
+        for (int q = -t1g; q < t1n + t1g; q++) {
+            for (int a = -g; a < n + g; a++) line(a) = in(at(a, p, q));
+            sweep(line, dt / d);
+            for (int a = -g; a < n + g; a++) at(a, p, q) = in(line(a));  // the swaps are their own inverses
+        }
+    });
+}
+// Grid2D/3D::split_step (Splitting.cpp:118-216): Strang splitting, the sweep
+// order rotating with sweep_step; nothing is committed if a sweep throws.
+// Grid1D::split_step is unsplit_step (Splitting.cpp:98).
+void Grid::split_step(double dt) {
+    CFG = cfg;
+    if (dim == 1) { step1D(dt); return; }
+    if (cfg.mhd) throw std::runtime_error("MHD requires Unsplit dimensions. Please enable DIMENSION_UNSPLIT in Config.h");
+    const Arr<Prim> saved = w;
+    const double h = dt / 2;
+    try {
+        if (dim == 2) {
+            if (sweep_step % 2 == 0) { advance_axis(0, h); advance_axis(1, dt); advance_axis(0, h); }
+            else { advance_axis(1, h); advance_axis(0, dt); advance_axis(1, h); }
+        } else {
+            static const int order[6][3] = {{0, 1, 2}, {2, 0, 1}, {1, 2, 0}, {2, 1, 0}, {0, 2, 1}, {1, 0, 2}};
+            const int* o = order[sweep_step % 6];
+            advance_axis(o[0], h); advance_axis(o[1], h); advance_axis(o[2], dt); advance_axis(o[1], h); advance_axis(o[0], h);
+        }
+    } catch (...) {
+        w = saved;
+        throw;
+    }
+    sweep_step++;
+}
+
+// Grid::advance_unsplit / advance_split (CFL.cpp:160-238)
+long Grid::advance(double dt, bool check_cfl, long* failures, long max_steps, bool split) {
     long steps = 0, fails = 0;
     while (dt > cfg.timestep_tol) {
         if (max_steps >= 0 && steps >= max_steps) break;
@@ -1051,7 +1102,8 @@ long Grid::advance(double dt, bool check_cfl, long* failures, long max_steps) {
         double t1 = check_cfl ? std::min(dt, cfl_time()) : dt;
         while (true) {
             try {
-                unsplit_step(t1);
+                if (split) split_step(t1);
+                else unsplit_step(t1);
                 break;
             } catch (const std::exception&) {
                 fails++;

@@ -186,12 +186,18 @@ struct Grid {
     void apply_boundary();
     double cfl_time() const;
     void unsplit_step(double dt);   // throws std::runtime_error on failure (no commit)
-    // advance_unsplit (CFL.cpp:213-238).  Returns the number of accepted steps;
-    // throws if the timestep falls below tolerance (the reference exit(1)s).
-    long advance(double dt, bool check_cfl, long* failures = nullptr, long max_steps = -1);
+    // Strang split step (Splitting.cpp:118-216, hydro only); sweep_step picks the
+    // order of the directional sweeps and advances only on success.
+    void split_step(double dt);
+    int sweep_step = 0;
+    // advance_unsplit / advance_split (CFL.cpp:160-238).  Returns the number of
+    // accepted steps; throws if the timestep falls below tolerance (the reference
+    // exit(1)s).
+    long advance(double dt, bool check_cfl, long* failures = nullptr, long max_steps = -1, bool split = false);
     void initialize_B_fields();     // B_fields.cpp:117-149
   private:
     void step1D(double dt);
+    void advance_axis(int ax, double dt);  // advanceX/Y/Z (Splitting.cpp:219-299)
     void step2D(double dt);
     void step3D(double dt);
 };
