@@ -97,6 +97,7 @@ def lib():
             "dg_download_device": (C.c_int, [vp, vp, vp]),
             "dg_set_boundary": (C.c_int, [vp, C.POINTER(_BC), C.c_int]),
             "dg_apply_boundary": (C.c_int, [vp]),
+            "dg_apply_boundary_element": (C.c_int, [vp, C.POINTER(_BC)]),
             "dg_cfl_dt": (C.c_int, [vp, _dp, C.POINTER(_Status)]),
             "dg_step": (C.c_int, [vp, C.c_double, C.POINTER(_Status)]),
             "dg_advance": (C.c_int, [vp, C.c_double, C.c_int, C.c_long, C.POINTER(_Stats), C.POINTER(_Status)]),
@@ -184,6 +185,13 @@ def _faces(f) -> int:
 
 
 class GhostFill:
+    """A boundary element (Boundary/GhostFill.hpp:16-30).  The built-in kinds
+    run on the GPU.  A user subclass that overrides apply(grid) -- like the
+    reference's Jet (Examples/CollidingFlows/Jets/Jets.hpp:12-39) -- runs on
+    the host: the grid's state is downloaded into grid.w (cells, ghosts
+    included, 8 fields) and grid.B (faces), apply() edits their ghost entries,
+    and the state is uploaded again.  Lists mixing both keep the reference's
+    order (implicit Outflow on uncovered faces first, then the elements)."""
     kind = -1
 
     def __init__(self, faces=X | Y | Z, corner_ghosts=True):
@@ -194,6 +202,14 @@ class GhostFill:
 
     def get_faces(self):
         return self.faces
+
+    def apply(self, grid):
+        """Fill this element's ghost cells of grid.w / grid.B (host arrays)."""
+        raise NotImplementedError("GhostFill subclasses evaluated on the host override apply(grid)")
+
+    @property
+    def on_host(self) -> bool:
+        return type(self).apply is not GhostFill.apply
 
     def _c(self) -> _BC:
         b = _BC()
@@ -370,8 +386,14 @@ class Grid:
     def boundary(self, b):
         self._boundary = b if isinstance(b, BoundaryList) else BoundaryList(b)
 
+    def _host_bc(self) -> bool:
+        return any(e.on_host for e in self._boundary.elements)
+
     def _push_bc(self):
-        els = [e._c() for e in self._boundary.elements]
+        # with host elements the device keeps a list of Ignore on every face, so
+        # the library's own boundary passes (dg_init_B, the split step) do nothing
+        # and apply_boundary() sequences the list itself
+        els = [Ignore(X | Y | Z)._c()] if self._host_bc() else [e._c() for e in self._boundary.elements]
         key = [(b.kind, b.faces, b.corners, b.gated, tuple(b.state)) for b in els]
         if key == self._bc_pushed:
             return
@@ -380,8 +402,39 @@ class Grid:
         self._bc_pushed = key
 
     def apply_boundary(self):
+        """BoundaryList::apply (BoundaryList.hpp:79-104)."""
         self._push_bc()
-        self._chk(lib().dg_apply_boundary(self._h))
+        if not self._host_bc():
+            self._chk(lib().dg_apply_boundary(self._h))
+            return
+        els = self._boundary.elements
+        covered = 0
+        for e in els:
+            covered |= e.faces
+        fresh = dirty = False  # host arrays current / edited since the download
+
+        def device(e):
+            nonlocal fresh, dirty
+            if dirty:
+                self.set_state(self.w, self.B if self.has_B else None)
+                dirty = False
+            fresh = False
+            b = e._c()
+            self._chk(lib().dg_apply_boundary_element(self._h, C.byref(b)))
+
+        device(Outflow(63 & ~covered))  # implicit Outflow on the uncovered faces first
+        for e in els:
+            if e.on_host:
+                if not fresh:
+                    self.w = self.get_w()
+                    self.B = self.get_B() if self.has_B else None
+                    fresh = True
+                e.apply(self)
+                dirty = True
+            else:
+                device(e)
+        if dirty:
+            self.set_state(self.w, self.B if self.has_B else None)
 
     # -- stepping
     def _chk(self, rc, st: _Status | None = None):
@@ -405,9 +458,39 @@ class Grid:
     def on_step_fail(self, e) -> bool:  # Unsplit.cpp:21-23 (root grid: retry)
         return True
 
+    def _advance_host_bc(self, dt, check_cfl, max_steps, step):
+        """The advance loop (CFL.cpp:160-238) run from the host, for boundary
+        lists with host-evaluated elements."""
+        steps = fails = 0
+        t_adv = 0.0
+        tol = self.config.timestep_tol
+        while dt > tol:
+            if 0 <= max_steps <= steps:
+                break
+            self.apply_boundary()
+            t1 = min(dt, self.cfl_time()) if check_cfl else dt
+            while True:
+                try:
+                    step(t1)
+                    break
+                except DragonError as e:
+                    if e.code in (DG_ERR_HIP, DG_ERR_ARG, DG_ERR_UNSUPPORTED) or not self.on_step_fail(e):
+                        raise
+                fails += 1
+                t1 *= 0.5
+                if t1 < tol:
+                    raise DragonError(DG_ERR_DT_UNDERFLOW, "Timestep has fallen below minimum. Exiting")
+            dt -= t1
+            t_adv += t1
+            steps += 1
+        self.t_advanced = t_adv
+        return steps, fails
+
     def advance_unsplit(self, dt: float, check_cfl=True, max_steps=-1):
         """Grid::advance_unsplit (CFL.cpp:213-238); returns (steps, failed attempts)."""
         self._push_bc()
+        if self._host_bc():
+            return self._advance_host_bc(dt, check_cfl, max_steps, self.unsplit_step)
         st, S = _Status(), _Stats()
         self._chk(lib().dg_advance(self._h, float(dt), int(check_cfl), int(max_steps), C.byref(S), C.byref(st)), st)
         self.t_advanced = float(S.t_advanced)  # simulated time of this call
@@ -422,6 +505,9 @@ class Grid:
     def advance_split(self, dt: float, check_cfl=True, max_steps=-1):
         """Grid::advance_split (CFL.cpp:160-184); returns (steps, failed attempts)."""
         self._push_bc()
+        if self._host_bc():
+            raise DragonError(DG_ERR_UNSUPPORTED, "the split step applies its boundaries between sweeps on the device; "
+                                                  "host-evaluated GhostFill elements need the unsplit step")
         st, S = _Status(), _Stats()
         self._chk(lib().dg_advance_split(self._h, float(dt), int(check_cfl), int(max_steps), C.byref(S), C.byref(st)), st)
         self.t_advanced = float(S.t_advanced)
@@ -434,7 +520,11 @@ class Grid:
         return self.advance_split(dt, check_cfl, max_steps)
 
     def initialize_B_fields(self):
+        """initialize_B_fields (B_fields.cpp:117-149): the boundary pass, then
+        the cell-centred B from the faces."""
         self._push_bc()
+        if self._host_bc():
+            self.apply_boundary()  # the device list is Ignore, so dg_init_B's own pass does nothing
         self._chk(lib().dg_init_B(self._h))
 
     def timing(self, enable: int = -1):

@@ -354,6 +354,20 @@ static void apply_one(dg_grid* g, const dg_bc& b) {
     }
 }
 
+// One element applied alone, without the list's implicit Outflow: the step
+// by which a caller interleaves device elements with elements it evaluates
+// itself on the host (user GhostFill subclasses).
+int dg_apply_boundary_element(dg_grid* g, const dg_bc* bc) {
+    if (!g || !bc) return DG_ERR_ARG;
+    dg_bc b = *bc;
+    if (b.kind < 0 || b.kind > DG_BC_IGNORE) return fail(g, DG_ERR_ARG, "unknown boundary kind");
+    if (b.kind == DG_BC_PERIODIC) b.faces = ((b.faces & 3) ? 3 : 0) | ((b.faces & 12) ? 12 : 0) | ((b.faces & 48) ? 48 : 0);
+    apply_one(g, b);  // ghosts only: the apply pass's CFL maxima stay valid, as for dg_apply_boundary
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return fail(g, DG_ERR_HIP, hipGetErrorString(e));
+    return DG_OK;
+}
+
 int dg_apply_boundary(dg_grid* g) {
     if (!g) return DG_ERR_ARG;
     int uncovered = 63;  // BoundaryList.hpp:104-110

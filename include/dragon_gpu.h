@@ -117,6 +117,11 @@ int dg_download_device(dg_grid* g, double* w_soa, double* B_soa);
 int dg_set_boundary(dg_grid* g, const dg_bc* list, int n);
 /* boundary.apply(grid) (BoundaryList.hpp:84-93) */
 int dg_apply_boundary(dg_grid* g);
+/* One boundary element alone, without the implicit Outflow of the list (the
+ * element's apply(), e.g. Outflow.cpp / Periodic.cpp): lets a caller keep the
+ * reference's list order when some elements are user GhostFill subclasses
+ * evaluated on the host (Boundary/GhostFill.hpp:16-30; BoundaryList.hpp:79-104). */
+int dg_apply_boundary_element(dg_grid* g, const dg_bc* bc);
 /* CFL::cfl_time(grid) (Hydro/CFL/CFL.cpp:113-162) */
 int dg_cfl_dt(dg_grid* g, double* dt, dg_status* st);
 /* GridND::unsplit_step(dt) (Unsplit.cpp:28-206, Splitting.cpp:100-115): no commit on failure. */

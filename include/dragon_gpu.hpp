@@ -119,16 +119,54 @@ struct Ignore : GhostFill {
     explicit Ignore(int faces = X | Y | Z) { bc = make_fill(DG_BC_IGNORE, faces, true).bc; }
     explicit Ignore(const std::string& f) : Ignore(face_mask(f)) {}
 };
+class GpuGrid;
+// A boundary element evaluated on the host: the counterpart of a user
+// GhostFill subclass (Boundary/GhostFill.hpp:16-30; e.g. the reference's Jet,
+// Examples/CollidingFlows/Jets/Jets.hpp:12-39).  apply() edits the ghost
+// entries of grid.cell(...) / grid.face(...); the grid downloads its state
+// before and uploads it after, keeping the list's order with the device
+// elements around it.  Not usable with the split step (its boundary passes
+// run on the device between sweeps).
+struct HostGhostFill {
+    int faces;
+    bool corners;
+    explicit HostGhostFill(int faces_ = X | Y | Z, bool corner_ghosts = true) : faces(faces_), corners(corner_ghosts) {}
+    virtual ~HostGhostFill() = default;
+    int get_faces() const { return faces; }
+    virtual void apply(GpuGrid& grid) = 0;
+};
 struct BoundaryList {
     std::vector<dg_bc> items;
+    std::vector<std::shared_ptr<HostGhostFill>> host;  // per item: null for a device element
     BoundaryList() = default;
     template <class... T>
-    explicit BoundaryList(const T&... e) { (items.push_back(e.bc), ...); }
-    BoundaryList& operator+=(const GhostFill& e) { items.push_back(e.bc); return *this; }
-    BoundaryList& operator+=(const BoundaryList& o) { items.insert(items.end(), o.items.begin(), o.items.end()); return *this; }
-    BoundaryList(const GhostFill& e) { items.push_back(e.bc); }  // NOLINT: implicit, as the reference's
+    explicit BoundaryList(const T&... e) { (add(e), ...); }
+    BoundaryList& operator+=(const GhostFill& e) { add(e); return *this; }
+    BoundaryList& operator+=(std::shared_ptr<HostGhostFill> e) { add(std::move(e)); return *this; }
+    BoundaryList& operator+=(const BoundaryList& o) {
+        items.insert(items.end(), o.items.begin(), o.items.end());
+        host.insert(host.end(), o.host.begin(), o.host.end());
+        return *this;
+    }
+    BoundaryList(const GhostFill& e) { add(e); }  // NOLINT: implicit, as the reference's
+    bool has_host() const {
+        for (const auto& h : host)
+            if (h) return true;
+        return false;
+    }
+
+private:
+    void add(const GhostFill& e) { items.push_back(e.bc); host.push_back(nullptr); }
+    void add(std::shared_ptr<HostGhostFill> e) {
+        dg_bc b{};
+        b.kind = DG_BC_IGNORE;
+        b.faces = e->faces;
+        items.push_back(b);
+        host.push_back(std::move(e));
+    }
 };
 inline BoundaryList operator+(BoundaryList a, const GhostFill& b) { a += b; return a; }
+inline BoundaryList operator+(BoundaryList a, std::shared_ptr<HostGhostFill> b) { a += std::move(b); return a; }
 inline BoundaryList operator+(const GhostFill& a, const GhostFill& b) { BoundaryList l(a); l += b; return l; }
 
 // Grid::advance / advance_unsplit / unsplit_step / on_step_fail / initialize_B_fields.
@@ -177,9 +215,33 @@ public:
         check(dg_upload(h_.get(), w_.data(), 8, has_B_ ? B_.data() : nullptr));
     }
 
+    // BoundaryList::apply (BoundaryList.hpp:79-104): implicit Outflow on the
+    // uncovered faces, then the elements in order
     void apply_boundary() {
         push_bc();
-        check(dg_apply_boundary(h_.get()));
+        if (!boundary.has_host()) {
+            check(dg_apply_boundary(h_.get()));
+            return;
+        }
+        int covered = 0;
+        for (const auto& b : boundary.items) covered |= b.faces;
+        bool fresh = false, dirty = false;  // host mirror current / edited since the download
+        auto device = [&](const dg_bc& b) {
+            if (dirty) { push(); dirty = false; }
+            fresh = false;
+            check(dg_apply_boundary_element(h_.get(), &b));
+        };
+        device(Outflow(63 & ~covered).bc);
+        for (size_t q = 0; q < boundary.items.size(); q++) {
+            if (boundary.host[q]) {
+                if (!fresh) { pull(); fresh = true; }
+                boundary.host[q]->apply(*this);
+                dirty = true;
+            } else {
+                device(boundary.items[q]);
+            }
+        }
+        if (dirty) push();
     }
     double cfl_time() {
         push_bc();
@@ -190,6 +252,7 @@ public:
     }
     void initialize_B_fields() {
         push_bc();
+        if (boundary.has_host()) apply_boundary();  // the device list is Ignore: dg_init_B's own pass does nothing
         check(dg_init_B(h_.get()));
     }
     // Grid::unsplit_step(dt): throws on failure, commits nothing
@@ -214,9 +277,12 @@ public:
 protected:
     long advance_loop(double dt, bool check_cfl, bool split) {
         push_bc();
+        if (split && boundary.has_host())
+            throw GpuError(DG_ERR_UNSUPPORTED, "the split step applies its boundaries on the device between sweeps; "
+                                               "host-evaluated GhostFill elements need the unsplit step");
         long steps = 0;
         while (dt > tol_) {
-            check(dg_apply_boundary(h_.get()));
+            apply_boundary();
             double t1 = dt;
             if (check_cfl) {
                 const double c = cfl_time();
@@ -245,6 +311,13 @@ protected:
         return ((long)(i + g_) * ey + (dim_ >= 2 ? j + g_ : 0)) * ez + (dim_ >= 3 ? k + g_ : 0);
     }
     void push_bc() {
+        if (boundary.has_host()) {  // the library's own boundary passes must do nothing; apply_boundary() sequences
+            dg_bc ign{};
+            ign.kind = DG_BC_IGNORE;
+            ign.faces = 63;
+            check(dg_set_boundary(h_.get(), &ign, 1));
+            return;
+        }
         check(dg_set_boundary(h_.get(), boundary.items.data(), (int)boundary.items.size()));
     }
     void check(int rc, const dg_status* st = nullptr) const {

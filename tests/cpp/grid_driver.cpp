@@ -4,6 +4,9 @@
 //
 //   grid_driver abi
 //       checks that need no GPU: bad descriptions throw GpuError(DG_ERR_ARG).
+//   grid_driver hostbc
+//       a host-evaluated boundary element against the built-in one it restates,
+//       the split step's refusal of host elements, advance() in split mode.
 //   grid_driver run DIM NX NY NZ DX DY DZ MHD SOLVER LIMITER CT_ENERGY GAMMA BC T_END IN OUT
 //       BC: periodic | outflow.  IN holds the cells (ghosts included, 8
 //       doubles each, grid order) and, for MHD in 2D/3D, the faces (3 doubles
@@ -51,8 +54,87 @@ static int abi_checks() {
     return failures ? 1 : 0;
 }
 
+// Fixed.cpp's cell fill with corner ghosts, written as a host element.
+struct HostFixed : HostGhostFill {
+    State st;
+    HostFixed(const State& s, int faces) : HostGhostFill(faces, true), st(s) {}
+    void apply(GpuGrid& grid) override {
+        const int G = grid.getGhosts(), nx = grid.getSizeX(), ny = grid.getSizeY();
+        for (int j = -G; j < ny + G; j++)
+            for (int i = 1; i <= G; i++) {
+                if (faces & X_negative) grid.cell(-i, j) = st;
+                if (faces & X_positive) grid.cell(nx - 1 + i, j) = st;
+            }
+    }
+};
+
+// A host element in a boundary list gives the bits of the built-in element it
+// restates; the split step refuses host elements; Config.dimension_unsplit = 0
+// makes advance() the split advance.
+static int hostbc_checks() {
+    Config c;
+    c.mhd = 0;
+    c.limiter = DG_VANLEER;
+    const int n = 48;
+    auto fill = [&](GpuGrid2D& g) {
+        for (int i = 0; i < n; i++)
+            for (int j = 0; j < n; j++) {
+                State w;
+                w.rho = 1.0 + 0.3 * (j > n / 2);
+                w.p = 2.5;
+                w.vx = j > n / 2 ? 0.4 : -0.4;
+                w.vy = 0.01 * ((i * 7 + j * 3) % 11 - 5);
+                g.cell(i, j) = w;
+            }
+        g.push();
+    };
+    State inflow;
+    inflow.rho = 1.5; inflow.p = 2.5; inflow.vx = 0.3;
+    GpuGrid2D a(n, n, 1.0 / n, 1.0 / n, 2, c), b(n, n, 1.0 / n, 1.0 / n, 2, c);
+    fill(a);
+    fill(b);
+    a.boundary = Periodic(Y) + Fixed(inflow, X_negative) + Outflow(X_positive);
+    b.boundary = BoundaryList(Periodic(Y)) + std::make_shared<HostFixed>(inflow, X_negative) + Outflow(X_positive);
+    const long sa = a.advance(0.02), sb = b.advance(0.02);
+    a.pull();
+    b.pull();
+    int failures = 0;
+    const int G = a.getGhosts();
+    for (int i = -G; i < n + G; i++)
+        for (int j = -G; j < n + G; j++)
+            if (std::memcmp(&a.cell(i, j), &b.cell(i, j), sizeof(State))) failures++;
+    std::printf("host element vs Fixed: %ld / %ld steps, %d cells differ\n", sa, sb, failures);
+    if (sa != sb) failures++;
+    try {
+        b.advance_split(0.01);
+        std::printf("FAIL split step accepted a host element\n");
+        failures++;
+    } catch (const GpuError& e) {
+        if (e.code != DG_ERR_UNSUPPORTED) failures++;
+    }
+    Config cs = c;
+    cs.dimension_unsplit = 0;
+    GpuGrid2D s1(n, n, 1.0 / n, 1.0 / n, 2, cs), s2(n, n, 1.0 / n, 1.0 / n, 2, c);
+    fill(s1);
+    fill(s2);
+    s1.boundary = Periodic();
+    s2.boundary = Periodic();
+    const long k1 = s1.advance(0.02), k2 = s2.advance_split(0.02);
+    s1.pull();
+    s2.pull();
+    int dsplit = 0;
+    for (int i = -G; i < n + G; i++)
+        for (int j = -G; j < n + G; j++)
+            if (std::memcmp(&s1.cell(i, j), &s2.cell(i, j), sizeof(State))) dsplit++;
+    std::printf("advance() with dimension_unsplit = 0 vs advance_split: %ld / %ld steps, %d cells differ\n", k1, k2, dsplit);
+    failures += dsplit + (k1 != k2);
+    std::printf(failures ? "hostbc checks failed\n" : "hostbc checks passed\n");
+    return failures ? 1 : 0;
+}
+
 int main(int argc, char** argv) {
     if (argc >= 2 && !std::strcmp(argv[1], "abi")) return abi_checks();
+    if (argc >= 2 && !std::strcmp(argv[1], "hostbc")) return hostbc_checks();
     if (argc != 18 || std::strcmp(argv[1], "run")) {
         std::fprintf(stderr, "usage: grid_driver abi | run DIM NX NY NZ DX DY DZ MHD SOLVER LIMITER CT_ENERGY GAMMA BC T_END IN OUT\n");
         return 2;

@@ -66,3 +66,15 @@ def test_cpp_api_orszag_tang_matches_oracle(tmp_path):
     og = oracle_grid(pb)
     og.advance(0.02)
     assert np.array_equal(w, og.get_w()) and np.array_equal(B, og.get_B())
+
+
+@pytest.mark.gpu
+def test_cpp_api_host_boundary_element_and_split_mode():
+    """HostGhostFill in a BoundaryList (the C++ counterpart of a user GhostFill
+    subclass) gives the same bits as the built-in Fixed it restates; the split
+    step refuses it; Config.dimension_unsplit = 0 makes advance() split."""
+    need_driver()
+    r = subprocess.run([DRIVER, "hostbc"], capture_output=True, text=True, timeout=300)
+    print(r.stdout)
+    assert r.returncode == 0, r.stdout + r.stderr
+    assert "hostbc checks passed" in r.stdout
