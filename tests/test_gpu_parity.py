@@ -86,3 +86,35 @@ def test_mhd_3d_rectangular():
     P.interior(B, g, shape, extra=1)[..., 0] = 0.2
     cfg = dict(mhd=1, gamma=5.0 / 3.0, solver=5, limiter=2, ctu=1, ct_energy=2)
     run_pair((shape, (1 / 40, 1 / 9, 1 / 70), cfg, [(1, 63)], w, B, {"init_B": True}), steps=5)
+
+
+def fallback_case(name, n, dim, try_hlle):
+    """Riemann problems run at CFL 1.5, where HLLC's fluxes fail the verify
+    check of Flux.cpp:17-97 (L - F dt/dx or R + F dt/dx unphysical) on some
+    faces: the HLLE / exact-solver fallback runs, and some attempts fail and
+    halve dt.  Toro's test 3 (p 1000 | 0.01) and Einfeldt's 1-2-3 problem
+    (v -/+2, p 0.4).  dim 2: the same data along x, periodic in y."""
+    L, R = {"toro3": ((1.0, 1000.0, 0.0), (1.0, 0.01, 0.0)), "123": ((1.0, 0.4, -2.0), (1.0, 0.4, 2.0))}[name]
+    shape = (n,) if dim == 1 else (n, 8)
+    g = P.ghosts_for(False)
+    w, _ = P.empty_state(shape, g, False)
+    x = (np.arange(n) + 0.5) / n
+    wi = P.interior(w, g, shape)
+    for q in range(3):
+        col = np.where(x < 0.5, L[q], R[q])
+        wi[..., (0, 1, 2)[q]] = col if dim == 1 else col[:, None]
+    cfg = dict(mhd=0, gamma=5.0 / 3.0, solver=5, limiter=0, ctu=1, verify_fallback=1, fallback_try_hlle=int(try_hlle),
+               cfl_coeff=1.5)
+    bcs = [] if dim == 1 else [(0, 3), (1, 12)]
+    d = (1.0 / n,) if dim == 1 else (1.0 / n, 1.0 / n)
+    return shape, d, cfg, bcs, w, None, {}
+
+
+@pytest.mark.parametrize("name,dim,try_hlle", [("toro3", 1, False), ("toro3", 1, True), ("123", 1, False), ("toro3", 2, False),
+                                               ("123", 2, True)])
+def test_verify_fallback_paths(name, dim, try_hlle):
+    """The verify-fallback chain (HLLC -> [HLLE] -> exact, Flux.cpp:17-97) and
+    the dt halving of rejected attempts on the grid, bit for bit with the
+    oracle, 1D and 2D CTU, with and without the HLLE stage."""
+    res = run_pair(fallback_case(name, 200, dim, try_hlle), steps=40, nf=5)
+    assert res["bit_equal"]
