@@ -100,3 +100,21 @@ def test_create_rejects_bad_descriptor(bad):
     assert D.lib().dg_create(C.byref(d), C.byref(out)) == D.DG_ERR_ARG
     assert not out.value
     assert D.lib().dg_create(None, C.byref(out)) == D.DG_ERR_ARG
+
+
+def test_product_path_fails_loudly_without_the_library(tmp_path):
+    """No CPU fallback: with libdragon_gpu.so absent, creating a grid raises
+    ImportError instead of computing anything elsewhere."""
+    import subprocess
+    import sys
+    code = ("import dragon_amd as D\n"
+            f"D.LIB_PATH = {str(tmp_path / 'absent.so')!r}\n"
+            "try:\n"
+            "    D.Grid2D(8, 8, 0.1, 0.1, 1, D.Config(mhd=0))\n"
+            "except ImportError as e:\n"
+            "    print('ImportError', e)\n"
+            "else:\n"
+            "    print('no error')\n")
+    r = subprocess.run([sys.executable, "-c", code], capture_output=True, text=True, timeout=120,
+                       cwd=os.path.dirname(os.path.dirname(os.path.abspath(__file__))))
+    assert "ImportError" in r.stdout and "absent.so" in r.stdout, r.stdout + r.stderr
