@@ -37,7 +37,7 @@ enum {
     DG_ERR_DT_UNDERFLOW = 6,      /* "Timestep has fallen below minimum."  CFL.cpp:230-233 (reference exits) */
     DG_ERR_ARG = 7,
     DG_ERR_HIP = 8,
-    DG_ERR_UNSUPPORTED = 9        /* entry point switched off in this build (dg_last_error / dg_batch_last_error says why) */
+    DG_ERR_UNSUPPORTED = 9        /* operation not available for this grid, e.g. the split step on MHD (dg_last_error says why) */
 };
 
 /* Config.h switches (Config.h:27-110) and the grid geometry (Grid.hpp:46,70,112). */

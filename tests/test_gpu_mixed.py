@@ -1,8 +1,8 @@
 """Mixed-precision form (FP32 reconstruction, limiters and Riemann solves;
 FP64 state, accumulation, pressure recovery and CT) against the FP64 oracle.
 
-Bar: relative L1(rho) <= 1e-5 against the FP64
-oracle, normalised max|div B| <= 1e-12.
+Bar of the mixed form: relative L1(rho) <= 1e-5 against the FP64 oracle,
+normalised max|div B| <= 1e-12.
 """
 import numpy as np
 import pytest

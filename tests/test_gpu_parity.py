@@ -1,6 +1,6 @@
 """HIP path vs the CPU oracle on the same seeded inputs (FP64 drop-in).
 
-Bar: relative L1 per variable <= 1e-10 after the run.
+Bar of the FP64 drop-in: relative L1 per variable <= 1e-10 after the run.
 The FP64 kernels are built without FMA contraction, so in practice the
 results are expected to be bit-identical; the bit count is printed.
 """
@@ -67,7 +67,7 @@ def test_hydro_3d_ctu12():
 
 
 def test_fp64_parity_100_steps_2d_mhd():
-    """<= 1e-10 relative L1 per variable after 100 steps."""
+    """The drop-in bar: <= 1e-10 relative L1 per variable after 100 steps."""
     run_pair(P.orszag_tang_2d(48), steps=100)
 
 

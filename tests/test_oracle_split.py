@@ -135,7 +135,7 @@ def test_split_periodic_conservation(dim):
     is 1e-14; the 3D split energy total here drifts 1.4e-14 (the unsplit step:
     2.3e-15): every one of the five sweeps of a 3D split step makes a
     prim -> cons -> prim round trip per cell, so 90 steps make 450 of them, and
-    this build rounds without FMA contraction.  Asserted at 1e-14 in 2D and
+    the oracle is compiled without FMA contraction.  Asserted at 1e-14 in 2D and
     3e-14 in 3D."""
     shape = (10,) * dim
     G = O.Grid(shape, (1.0,) * dim, hydro(), ghosts=2)
