@@ -581,14 +581,15 @@ __device__ __forceinline__ void sweep_wave_body(const SweepArgs& a, long wave, i
     constexpr int T1 = D == 3 ? 1 : 0;    // faster transverse dim
     const int len = a.hi[AX] - a.lo[AX];
     const int nseg = (len + 62) / 63;
-    const long n1 = D == 3 ? a.hi[T1] - a.lo[T1] : 1;
-    const long n0 = a.hi[T0] - a.lo[T0];
+    const int n1 = D == 3 ? a.hi[T1] - a.lo[T1] : 1;
+    const int n0 = a.hi[T0] - a.lo[T0];
     if (wave >= (long)nseg * n0 * n1) return;  // uniform per wavefront
     int idx[3] = {0, 0, 0};
-    const int seg = (int)(wave % nseg);
-    long r = wave / nseg;
-    if (D == 3) { idx[T1] = a.lo[T1] + (int)(r % n1); r /= n1; }
-    idx[T0] = a.lo[T0] + (int)r;
+    int r = (int)wave;  // < 2^31 wavefronts: 32-bit division (the host sizes grids far below)
+    const int seg = r % nseg;
+    r /= nseg;
+    if (D == 3) { idx[T1] = a.lo[T1] + r % n1; r /= n1; }
+    idx[T0] = a.lo[T0] + r;
     const int c = a.lo[AX] + seg * 63 - 1 + lane;
     const bool live = c < a.hi[AX];
     P<R> L = pzero<R>(), Rr = pzero<R>();
@@ -598,7 +599,12 @@ __device__ __forceinline__ void sweep_wave_body(const SweepArgs& a, long wave, i
     if (lane >= 1 && live) face_flux<MHD, R, AX>(a, Rl, L, a.ce.at(idx[0], idx[1], idx[2]));
 }
 template <int D, bool MHD, class R, int AX, bool FINAL>
-KS __global__ void k_sweep_wave(SweepArgs a) { sweep_wave_body<D, MHD, R, AX, FINAL>(a, tid() >> 6, threadIdx.x & 63); }
+KS __global__ void k_sweep_wave(SweepArgs a) {
+    // the wavefront index is uniform: read it into a scalar register so the
+    // index arithmetic of sweep_wave_body runs once per wavefront on the SALU
+    const int w = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+    sweep_wave_body<D, MHD, R, AX, FINAL>(a, (long)blockIdx.x * (blockDim.x >> 6) + w, threadIdx.x & 63);
+}
 
 // ===================================================== CTU hydro (2D/3D)
 // correctState (CTU.cpp:264-301): states along SAX corrected by the flux
@@ -1622,11 +1628,12 @@ KS __global__ void kb_sweep_x(BatchArgs A, BlockTable bt) {
 }
 template <bool FIN>
 KS __global__ void kb_sweep_y(BatchArgs A, BlockTable bt) {
-    BATCH_ITEM(bt)  // first is a multiple of the workgroup size, so t >> 6 is this wavefront
+    BATCH_ITEM(bt)  // first is a multiple of the workgroup size: first / 64 + the wavefront in the workgroup
     const PatchView v = patch_view(A, p);
     SweepArgs s = batch_sweep_args<1, FIN>(A, v, p);
     s.chunk = 63;
-    sweep_wave_body<2, true, RT, 1, FIN>(s, t >> 6, threadIdx.x & 63);
+    const long w = (long)bt.first[blockIdx.x] / 64 + __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+    sweep_wave_body<2, true, RT, 1, FIN>(s, w, threadIdx.x & 63);
 }
 template <bool PRED>
 KB __global__ void kb_emf(BatchArgs A, BlockTable bt) {
