@@ -478,15 +478,17 @@ template <class R> __device__ __forceinline__ P<double> p_wide(const P<R>& w) {
     return P<double>{double(w.rho), double(w.p), double(w.vx), double(w.vy), double(w.vz), double(w.Bx), double(w.By), double(w.Bz)};
 }
 
-// Half states of cell (i,j,k) along AX, in the global frame.
+// Half states of cell (i,j,k) along AX, in the global frame, from the cell
+// and its two neighbours along AX (wl, wr are read only for cells that are
+// not on the outer ghost layer).
 template <int D, bool MHD, class R, int AX, bool FINAL>
-__device__ __forceinline__ void cell_states(const SweepArgs& a, int i, int j, int k, P<R>& L, P<R>& Rr) {
+__device__ __forceinline__ void cell_states_from(const SweepArgs& a, int i, int j, int k, const P<R>& wl, const P<R>& wc,
+                                                 const P<R>& wr, P<R>& L, P<R>& Rr) {
     const Ext& ce = a.ce;
     const Ext& fe = a.fe;
     const long m = ce.at(i, j, k);
     const int idx[3] = {i, j, k};
     const int g = ce.g;
-    const P<R> wc = load_p<R>(a.W, m, MHD);
     double bx = 0, by = 0, bz = 0;
     long q = 0;
     if (MHD && D >= 2) {
@@ -496,8 +498,6 @@ __device__ __forceinline__ void cell_states(const SweepArgs& a, int i, int j, in
         if (D == 3) bz = (a.B[2][q + fe.s[2]] - a.B[2][q]) * a.dt_d[2];
     }
     if (idx[AX] >= -g + 1 && idx[AX] < ce.n[AX] + g - 1) {
-        const long off = ce.s[AX];
-        const P<R> wl = load_p<R>(a.W, m - off, MHD), wr = load_p<R>(a.W, m + off, MHD);
         if (AX == 0) {
             muscl<MHD, R>(wl, L, wc, Rr, wr, R(a.dt_d[0]), R(bx), R(by), R(bz), a.ph);
         } else if (AX == 1) {
@@ -527,6 +527,21 @@ __device__ __forceinline__ void cell_states(const SweepArgs& a, int i, int j, in
     }
 }
 
+// The same, reading the three cells.
+template <int D, bool MHD, class R, int AX, bool FINAL>
+__device__ __forceinline__ void cell_states(const SweepArgs& a, int i, int j, int k, P<R>& L, P<R>& Rr) {
+    const long m = a.ce.at(i, j, k);
+    const int c = AX == 0 ? i : AX == 1 ? j : k;
+    const P<R> wc = load_p<R>(a.W, m, MHD);
+    if (c >= -a.ce.g + 1 && c < a.ce.n[AX] + a.ce.g - 1) {
+        const long off = a.ce.s[AX];
+        const P<R> wl = load_p<R>(a.W, m - off, MHD), wr = load_p<R>(a.W, m + off, MHD);
+        cell_states_from<D, MHD, R, AX, FINAL>(a, i, j, k, wl, wc, wr, L, Rr);
+    } else {
+        cell_states_from<D, MHD, R, AX, FINAL>(a, i, j, k, wc, wc, wc, L, Rr);
+    }
+}
+
 template <bool MHD, class R, int AX>
 __device__ __forceinline__ void face_flux(const SweepArgs& a, P<R> l, P<R> r, long m) {
     if (AX == 1) { l = swapXY(l); r = swapXY(r); }
@@ -538,6 +553,13 @@ __device__ __forceinline__ void face_flux(const SweepArgs& a, P<R> l, P<R> r, lo
     if (AX == 2) f = swapXZ(f);
     store_u<R>(a.Fo, m, f, MHD);
 }
+
+// Windowed walkers (stencil cells carried in registers) for the FP32 form;
+// the FP64 kernels are register-bound and re-read the stencil.
+#ifndef DG_MARCH_WINDOW
+#define DG_MARCH_WINDOW 1
+#endif
+template <class R> constexpr bool march_window() { return DG_MARCH_WINDOW == 2 || (DG_MARCH_WINDOW == 1 && sizeof(R) == 4); }
 
 // Strided axis: one thread walks `chunk` consecutive faces.
 template <int D, bool MHD, class R, int AX, bool FINAL>
@@ -557,16 +579,47 @@ __device__ __forceinline__ void sweep_march_body(const SweepArgs& a, long t) {
     const int f1 = min(a.hi[AX], f0 + a.chunk);
     P<R> Lp, Rp, Lc, Rc;
     idx[AX] = f0 - 1;
-    cell_states<D, MHD, R, AX, FINAL>(a, idx[0], idx[1], idx[2], Lp, Rp);
-    for (int f = f0; f < f1; f++) {
-        idx[AX] = f;
-        cell_states<D, MHD, R, AX, FINAL>(a, idx[0], idx[1], idx[2], Lc, Rc);
-        face_flux<MHD, R, AX>(a, Rp, Lc, a.ce.at(idx[0], idx[1], idx[2]));
-        Rp = Rc;
+    if constexpr (march_window<R>()) {
+        // The three cells of the MUSCL stencil stay in registers as the
+        // walker advances; the cell two ahead is read (as FP64) before the
+        // Riemann solve of the current face and narrowed when it is used.
+        const long s = a.ce.s[AX];
+        const int clo = -a.ce.g, chi = a.ce.n[AX] + a.ce.g;  // cells that exist along AX
+        long m = a.ce.at(idx[0], idx[1], idx[2]);
+        auto ld = [&](int c, long mm) { return c >= clo && c < chi ? load_p<double>(a.W, mm, MHD) : pzero<double>(); };
+        P<R> wl = p_cast<R>(ld(f0 - 2, m - s)), wc = p_cast<R>(ld(f0 - 1, m));
+        P<R> wr = p_cast<R>(ld(f0, m + s));
+        cell_states_from<D, MHD, R, AX, FINAL>(a, idx[0], idx[1], idx[2], wl, wc, wr, Lp, Rp);
+        P<double> nxt = ld(f0 + 1, m + 2 * s);
+        for (int f = f0; f < f1; f++) {
+            idx[AX] = f;
+            m += s;
+            wl = wc;
+            wc = wr;
+            wr = p_cast<R>(nxt);
+            cell_states_from<D, MHD, R, AX, FINAL>(a, idx[0], idx[1], idx[2], wl, wc, wr, Lc, Rc);
+            nxt = ld(f + 2, m + 2 * s);
+            face_flux<MHD, R, AX>(a, Rp, Lc, m);
+            Rp = Rc;
+        }
+    } else {
+        cell_states<D, MHD, R, AX, FINAL>(a, idx[0], idx[1], idx[2], Lp, Rp);
+        for (int f = f0; f < f1; f++) {
+            idx[AX] = f;
+            cell_states<D, MHD, R, AX, FINAL>(a, idx[0], idx[1], idx[2], Lc, Rc);
+            face_flux<MHD, R, AX>(a, Rp, Lc, a.ce.at(idx[0], idx[1], idx[2]));
+            Rp = Rc;
+        }
     }
 }
+// Occupancy floor of the walkers: the FP32 predictor walker needs 133
+// registers, one wave per SIMD short of four; at four it spills 8 bytes per
+// lane and runs 5-8% faster.  1 = the compiler's choice.
+template <class R, bool FINAL> constexpr int march_min_waves() { return DG_SWEEP_WAVES == 0 && sizeof(R) == 4 && !FINAL ? 4 : 1; }
 template <int D, bool MHD, class R, int AX, bool FINAL>
-KS __global__ void k_sweep_march(SweepArgs a) { sweep_march_body<D, MHD, R, AX, FINAL>(a, tid()); }
+KS __attribute__((amdgpu_waves_per_eu(march_min_waves<R, FINAL>()))) __global__ void k_sweep_march(SweepArgs a) {
+    sweep_march_body<D, MHD, R, AX, FINAL>(a, tid());
+}
 
 template <class R> __device__ __forceinline__ P<R> shfl_up1(const P<R>& w) {
     return P<R>{__shfl_up(w.rho, 1), __shfl_up(w.p, 1), __shfl_up(w.vx, 1), __shfl_up(w.vy, 1),
