@@ -33,6 +33,7 @@ struct dg_grid {
     double *HS = nullptr, *F = nullptr, *Fp = nullptr, *T = nullptr;
     bool fused = true;
     int march_chunk = 0;
+    int march_window = 0;
     double* Ws = nullptr;       // split step: third state buffer (allocated on first use)
     int sweep_step = 0;         // split step: rotation of the sweep order (Grid.hpp:93, 135)
     double *E0 = nullptr, *Eh = nullptr;
@@ -126,6 +127,7 @@ static StepCtx make_ctx(dg_grid* g, double dt) {
     c.ctu = g->d.ctu; c.cfl_mode = g->d.cfl_mode;
     c.fused = g->fused;
     c.march_chunk = g->march_chunk;
+    c.march_window = g->march_window;
     c.ce = g->ce; c.fe = g->fe; c.ph = g->ph;
     c.d[0] = g->d.dx; c.d[1] = g->d.dim >= 2 ? g->d.dy : 1.0; c.d[2] = g->d.dim >= 3 ? g->d.dz : 1.0;
     c.dt = dt;
@@ -188,6 +190,9 @@ int dg_create(const dg_desc* desc, dg_grid** out) {
     g->fused = fusable && getenv("DG_UNFUSED") == nullptr;
     // diagnostic: fixed faces per walker of the strided-axis sweeps (results do not depend on it)
     if (const char* mc = getenv("DG_MARCH_CHUNK")) g->march_chunk = atoi(mc);
+    // check: the FP64 walkers take the register-window form the FP32 build
+    // uses, so the FP64 parity tests hold that form to the oracle bit for bit
+    g->march_window = getenv("DG_MARCH_WINDOW_CHECK") != nullptr;
     ok = ok && alloc(&g->F, (long)d.dim * g->nf * nc);
     if (g->fused) ok = ok && (!d.ctu || alloc(&g->Fp, (long)d.dim * g->nf * nc));
     else ok = ok && alloc(&g->HS, 2L * d.dim * g->nf * nc);

@@ -555,14 +555,14 @@ __device__ __forceinline__ void face_flux(const SweepArgs& a, P<R> l, P<R> r, lo
 }
 
 // Windowed walkers (stencil cells carried in registers) for the FP32 form;
-// the FP64 kernels are register-bound and re-read the stencil.
-#ifndef DG_MARCH_WINDOW
-#define DG_MARCH_WINDOW 1
-#endif
-template <class R> constexpr bool march_window() { return DG_MARCH_WINDOW == 2 || (DG_MARCH_WINDOW == 1 && sizeof(R) == 4); }
+// the FP64 walkers are register-bound (the window takes them from 236 to 256
+// VGPRs, one wave per SIMD) and re-read the stencil.  The windowed FP64
+// instantiation exists as a check of the window's cell bookkeeping: its
+// results must equal the oracle's bit for bit.
+template <class R> constexpr bool march_window() { return sizeof(R) == 4; }
 
 // Strided axis: one thread walks `chunk` consecutive faces.
-template <int D, bool MHD, class R, int AX, bool FINAL>
+template <int D, bool MHD, class R, int AX, bool FINAL, bool WIN = march_window<R>()>
 __device__ __forceinline__ void sweep_march_body(const SweepArgs& a, long t) {
     constexpr int T0 = AX == 0 ? 1 : 0;            // slower transverse dim (3D only)
     constexpr int T1 = D == 3 ? (AX == 2 ? 1 : 2) : (AX == 0 ? 1 : 0);  // fastest transverse dim
@@ -579,7 +579,7 @@ __device__ __forceinline__ void sweep_march_body(const SweepArgs& a, long t) {
     const int f1 = min(a.hi[AX], f0 + a.chunk);
     P<R> Lp, Rp, Lc, Rc;
     idx[AX] = f0 - 1;
-    if constexpr (march_window<R>()) {
+    if constexpr (WIN) {
         // The three cells of the MUSCL stencil stay in registers as the
         // walker advances; the cell two ahead is read (as FP64) before the
         // Riemann solve of the current face and narrowed when it is used.
@@ -620,6 +620,8 @@ template <int D, bool MHD, class R, int AX, bool FINAL>
 KS __attribute__((amdgpu_waves_per_eu(march_min_waves<R, FINAL>()))) __global__ void k_sweep_march(SweepArgs a) {
     sweep_march_body<D, MHD, R, AX, FINAL>(a, tid());
 }
+template <int D, bool MHD, class R, int AX, bool FINAL>
+KS __global__ void k_sweep_march_window(SweepArgs a) { sweep_march_body<D, MHD, R, AX, FINAL, true>(a, tid()); }
 
 template <class R> __device__ __forceinline__ P<R> shfl_up1(const P<R>& w) {
     return P<R>{__shfl_up(w.rho, 1), __shfl_up(w.p, 1), __shfl_up(w.vx, 1), __shfl_up(w.vy, 1),
@@ -1301,6 +1303,12 @@ static void sweep_t(const StepCtx& c, const Box& b, const FSoa& out, int stage, 
         if (chunk > len) chunk = len;
         nch = (len + chunk - 1) / chunk;
         a.chunk = (int)chunk;
+        if constexpr (!march_window<R>()) {
+            if (c.march_window) {
+                hipLaunchKernelGGL((k_sweep_march_window<D, MHD, R, AX, FINAL>), dim3(blocks(nch * ntrans)), dim3(256), 0, s, a);
+                return;
+            }
+        }
         hipLaunchKernelGGL((k_sweep_march<D, MHD, R, AX, FINAL>), dim3(blocks(nch * ntrans)), dim3(256), 0, s, a);
     }
 }

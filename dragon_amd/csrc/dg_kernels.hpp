@@ -24,6 +24,7 @@ struct StepCtx {
     int ctu, cfl_mode;
     bool fused;           // fused sweeps (no half-state arrays) where available
     int march_chunk;      // faces one thread walks in a strided-axis sweep (0: automatic)
+    int march_window;     // 1: FP64 walkers run the register-window form of the FP32 build (a check; same results)
     Ext ce, fe;
     Phys ph;
     double d[3];

@@ -71,10 +71,9 @@ def test_fp64_parity_100_steps_2d_mhd():
     run_pair(P.orszag_tang_2d(48), steps=100)
 
 
-def test_mhd_3d_rectangular():
+def rect_mhd_3d():
     """Non-cubic 3D MHD (CTU + HLLD + CT, van Leer) on 40 x 9 x 70 cells with a
-    random state: several i-runs, j-tiles and k-tiles of the blocked EMF pass,
-    partial tiles on every side."""
+    random state."""
     shape = (40, 9, 70)
     g = P.ghosts_for(True)
     w, B = P.empty_state(shape, g, True)
@@ -85,7 +84,34 @@ def test_mhd_3d_rectangular():
     wi[..., 2:5] = 0.5 * (rng.random(shape + (3,)) - 0.5)
     P.interior(B, g, shape, extra=1)[..., 0] = 0.2
     cfg = dict(mhd=1, gamma=5.0 / 3.0, solver=5, limiter=2, ctu=1, ct_energy=2)
-    run_pair((shape, (1 / 40, 1 / 9, 1 / 70), cfg, [(1, 63)], w, B, {"init_B": True}), steps=5)
+    return (shape, (1 / 40, 1 / 9, 1 / 70), cfg, [(1, 63)], w, B, {"init_B": True})
+
+
+def test_mhd_3d_rectangular():
+    """rect_mhd_3d: several i-runs, j-tiles and k-tiles of the blocked EMF pass,
+    partial tiles on every side."""
+    run_pair(rect_mhd_3d(), steps=5)
+
+
+@pytest.mark.parametrize("case,chunk", [("c2", 7), ("c3", 7), ("c4", 5), ("rect", 7), ("rect", 64)])
+def test_windowed_walkers_exact(case, chunk, monkeypatch):
+    """The FP32 build's strided-axis walkers keep the three MUSCL cells in
+    registers and read the cell two ahead before each Riemann solve.  The same
+    code instantiated for FP64 (DG_MARCH_WINDOW_CHECK) must return the oracle's
+    bits: an exact check of the window's cell bookkeeping (first cell of a
+    walker, outer ghost layers, partial last walkers) that the tolerance of the
+    mixed form could not give.  Short walkers (DG_MARCH_CHUNK) put many walker
+    starts and a partial walker on every line."""
+    monkeypatch.setenv("DG_MARCH_WINDOW_CHECK", "1")
+    monkeypatch.setenv("DG_MARCH_CHUNK", str(chunk))
+    if case == "c2":
+        run_pair(P.kelvin_helmholtz_2d(64), steps=20, nf=5)
+    elif case == "c3":
+        run_pair(P.orszag_tang_2d(48), steps=20)
+    elif case == "c4":
+        run_pair(P.turbulence_3d(16), steps=6)
+    else:
+        run_pair(rect_mhd_3d(), steps=4)
 
 
 def fallback_case(name, n, dim, try_hlle):
