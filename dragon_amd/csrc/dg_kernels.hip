@@ -27,12 +27,17 @@ namespace DG_NS {
 #if DG_PREC == 32
 using RT = float;   // arithmetic type of reconstruction and Riemann solves
 using FT = float;   // storage type of the flux arrays (FP32 Riemann fluxes lose nothing)
+using ET = float;   // storage type of the corner and reference EMF arrays (the arithmetic on them stays FP64)
 #else
 using RT = double;
 using FT = double;
+using ET = double;
 #endif
 using FSoa = SoaT<FT>;
 __host__ __device__ inline FSoa fs(const Soa& a) { return FSoa{reinterpret_cast<FT*>(a.p), a.stride}; }
+__host__ __device__ inline ET* et(double* p) { return reinterpret_cast<ET*>(p); }
+// EMF element widened to FP64 before any arithmetic
+__device__ __forceinline__ double ed(const ET* p, long i) { return double(p[i]); }
 void ctu_hydro3(const StepCtx& c, hipStream_t s);
 
 #define KB __launch_bounds__(256)
@@ -392,7 +397,7 @@ struct SweepArgs {
     Soa W;
     const double* B[3];
     const double* Bh[3];
-    const double* E0[3];
+    const ET* E0[3];
     FSoa Fp[3];            // predictor fluxes (corrector pass input)
     FSoa Fo;               // output fluxes of this axis
     Ext ce, fe;
@@ -422,7 +427,7 @@ __device__ __forceinline__ void ctu_correct(const SweepArgs& a, long m, long q, 
         const double k8 = ph.k1_8pi;
         const P<double> Wc = load_p<double>(a.W, m, true);
         const double dt_dx = a.dt_d[0], dt_dy = a.dt_d[1], dt_dz = a.dt_d[2];
-        const double *E0x = a.E0[0], *E0y = a.E0[1], *E0z = a.E0[2];
+        const ET *E0x = a.E0[0], *E0y = a.E0[1], *E0z = a.E0[2];
         U<double> corr;
         if (D == 2) {
             if (AX == 0) {
@@ -430,35 +435,35 @@ __device__ __forceinline__ void ctu_correct(const SweepArgs& a, long m, long q, 
                 corr.Bx = corr.By = corr.Bz = 0;
                 double lim_z = minmod(-dBy, dBx);
                 corr.E += k8 * (Wc.Bz * Wc.vz * lim_z);
-                corr.Bz = 0.5 * dt_dy * (E0x[q + fy] - E0x[q]) + 0.5 * Wc.vz * lim_z;
+                corr.Bz = 0.5 * dt_dy * (ed(E0x, q + fy) - ed(E0x, q)) + 0.5 * Wc.vz * lim_z;
             } else {
                 corr = umul<true>(umul<true>(fd(0), 0.5), dt_dx);
                 corr.Bx = corr.By = corr.Bz = 0;
                 double lim_z = minmod(-dBx, dBy);
                 corr.E += k8 * (Wc.Bz * Wc.vz * lim_z);
-                corr.Bz = -0.5 * dt_dx * (E0y[q + fx] - E0y[q]) + 0.5 * Wc.vz * lim_z;
+                corr.Bz = -0.5 * dt_dx * (ed(E0y, q + fx) - ed(E0y, q)) + 0.5 * Wc.vz * lim_z;
             }
         } else if (AX == 0) {
             corr = umul<true>(uadd<true>(umul<true>(fd(1), dt_dy), umul<true>(fd(2), dt_dz)), 0.5);
             corr.Bx = corr.By = corr.Bz = 0;
             double lim_y = minmod(-dBz, dBx), lim_z = minmod(-dBy, dBx);
             corr.E += k8 * (Wc.By * Wc.vy * lim_y + Wc.Bz * Wc.vz * lim_z);
-            corr.By = -0.25 * dt_dz * (E0x[q + fz] - E0x[q] + E0x[q + fy + fz] - E0x[q + fy]) + 0.5 * Wc.vy * lim_y;
-            corr.Bz = 0.25 * dt_dy * (E0x[q + fy] - E0x[q] + E0x[q + fy + fz] - E0x[q + fz]) + 0.5 * Wc.vz * lim_z;
+            corr.By = -0.25 * dt_dz * (ed(E0x, q + fz) - ed(E0x, q) + ed(E0x, q + fy + fz) - ed(E0x, q + fy)) + 0.5 * Wc.vy * lim_y;
+            corr.Bz = 0.25 * dt_dy * (ed(E0x, q + fy) - ed(E0x, q) + ed(E0x, q + fy + fz) - ed(E0x, q + fz)) + 0.5 * Wc.vz * lim_z;
         } else if (AX == 1) {
             corr = umul<true>(uadd<true>(umul<true>(fd(0), dt_dx), umul<true>(fd(2), dt_dz)), 0.5);
             corr.Bx = corr.By = corr.Bz = 0;
             double lim_x = minmod(-dBz, dBy), lim_z = minmod(-dBx, dBy);
             corr.E += k8 * (Wc.Bx * Wc.vx * lim_x + Wc.Bz * Wc.vz * lim_z);
-            corr.Bx = 0.25 * dt_dz * (E0y[q + fz] - E0y[q] + E0y[q + fx + fz] - E0y[q + fx]) + 0.5 * Wc.vx * lim_x;
-            corr.Bz = -0.25 * dt_dx * (E0y[q + fx] - E0y[q] + E0y[q + fx + fz] - E0y[q + fz]) + 0.5 * Wc.vz * lim_z;
+            corr.Bx = 0.25 * dt_dz * (ed(E0y, q + fz) - ed(E0y, q) + ed(E0y, q + fx + fz) - ed(E0y, q + fx)) + 0.5 * Wc.vx * lim_x;
+            corr.Bz = -0.25 * dt_dx * (ed(E0y, q + fx) - ed(E0y, q) + ed(E0y, q + fx + fz) - ed(E0y, q + fz)) + 0.5 * Wc.vz * lim_z;
         } else {
             corr = umul<true>(uadd<true>(umul<true>(fd(0), dt_dx), umul<true>(fd(1), dt_dy)), 0.5);
             corr.Bx = corr.By = corr.Bz = 0;
             double lim_x = minmod(-dBy, dBz), lim_y = minmod(-dBx, dBz);
             corr.E += k8 * (Wc.Bx * Wc.vx * lim_x + Wc.By * Wc.vy * lim_y);
-            corr.Bx = -0.25 * dt_dy * (E0z[q + fy] - E0z[q] + E0z[q + fx + fy] - E0z[q + fx]) + 0.5 * Wc.vx * lim_x;
-            corr.By = 0.25 * dt_dx * (E0z[q + fx] - E0z[q] + E0z[q + fx + fy] - E0z[q + fy]) + 0.5 * Wc.vy * lim_y;
+            corr.Bx = -0.25 * dt_dy * (ed(E0z, q + fy) - ed(E0z, q) + ed(E0z, q + fx + fy) - ed(E0z, q + fx)) + 0.5 * Wc.vx * lim_x;
+            corr.By = 0.25 * dt_dx * (ed(E0z, q + fx) - ed(E0z, q) + ed(E0z, q + fx + fy) - ed(E0z, q + fy)) + 0.5 * Wc.vy * lim_y;
         }
         U<double> uL = uadd<true>(cons<true>(L, ph), corr), uR = uadd<true>(cons<true>(Rr, ph), corr);
         const double* Bh = a.Bh[AX];
@@ -719,14 +724,14 @@ enum { U_RHO = 0, U_BX = 5, U_BY = 6, U_BZ = 7 };
 // selects whether the cell state comes from W (predictor) or from the
 // half-step Eref arrays (corrector).
 struct ERef {
-    const double *x, *y, *z;
+    const ET *x, *y, *z;
 };
 
 // 2D corner EMF (Electric.cpp:24-35, 52-71) at edge (i,j).  With `fromW`,
 // Eref is computed from the cell states (predictor); otherwise it is read.
 template <bool FROM_W>
 __device__ __forceinline__ void emf2_body(const FSoa& FX, const FSoa& FY, const Soa& W, const ERef& ref, const Ext& ce, const Ext& fe,
-                                          const Box& box, int upwind, double* Ex, double* Ey, double* Ez, long t) {
+                                          const Box& box, int upwind, ET* Ex, ET* Ey, ET* Ez, long t) {
     int i, j, k;
     if (!box.point(t, i, j, k)) return;
     auto f = [&](const FSoa& F, int fld, int a, int b) { return double(F(fld, ce.at(a, b, 0))); };
@@ -742,7 +747,7 @@ __device__ __forceinline__ void emf2_body(const FSoa& FX, const FSoa& FY, const 
                     cross(W(F_BX, c), W(F_BY, c), W(F_BZ, c), W(F_VX, c), W(F_VY, c), W(F_VZ, c), rx[di][dj], ry[di][dj], rz[di][dj]);
                 } else {
                     const long c = fe.at(i - 1 + di, j - 1 + dj, 0);
-                    rx[di][dj] = ref.x[c]; ry[di][dj] = ref.y[c]; rz[di][dj] = ref.z[c];
+                    rx[di][dj] = ed(ref.x, c); ry[di][dj] = ed(ref.y, c); rz[di][dj] = ed(ref.z, c);
                 }
             }
         ez += upwind_corr(f(FX, U_RHO, i, j), f(FY, U_BX, i - 1, j), f(FY, U_BX, i, j), rz[0][1], rz[1][1]);
@@ -756,8 +761,8 @@ __device__ __forceinline__ void emf2_body(const FSoa& FX, const FSoa& FY, const 
     Ex[e] = ex; Ey[e] = ey; Ez[e] = ez;
 }
 template <bool FROM_W>
-KB __global__ void k_emf2(FSoa FX, FSoa FY, Soa W, ERef ref, Ext ce, Ext fe, Box box, int upwind, double* Ex, double* Ey,
-                          double* Ez) {
+KB __global__ void k_emf2(FSoa FX, FSoa FY, Soa W, ERef ref, Ext ce, Ext fe, Box box, int upwind, ET* Ex, ET* Ey,
+                          ET* Ez) {
     emf2_body<FROM_W>(FX, FY, W, ref, ce, fe, box, upwind, Ex, Ey, Ez, tid());
 }
 
@@ -794,8 +799,8 @@ __device__ __forceinline__ int emf_fld(int ax, int f) {
 
 // One thread per edge triple, every operand read from memory.
 template <bool FROM_W>
-KB __global__ void k_emf3(FSoa FX, FSoa FY, FSoa FZ, Soa W, ERef ref, Ext ce, Ext fe, Box box, int upwind, double* Ex,
-                          double* Ey, double* Ez) {
+KB __global__ void k_emf3(FSoa FX, FSoa FY, FSoa FZ, Soa W, ERef ref, Ext ce, Ext fe, Box box, int upwind, ET* Ex,
+                          ET* Ey, ET* Ez) {
     int i, j, k;
     if (!box.point(tid(), i, j, k)) return;
     auto fx = [&](int f, int a, int b, int c) { return double(FX(emf_fld(0, f), ce.at(i + a, j + b, k + c))); };
@@ -809,7 +814,7 @@ KB __global__ void k_emf3(FSoa FX, FSoa FY, FSoa FZ, Soa W, ERef ref, Ext ce, Ex
             return comp == 0 ? x : comp == 1 ? y : z;
         }
         const long q = fe.at(i + a, j + b, k + c);
-        return comp == 0 ? ref.x[q] : comp == 1 ? ref.y[q] : ref.z[q];
+        return comp == 0 ? ed(ref.x, q) : comp == 1 ? ed(ref.y, q) : ed(ref.z, q);
     };
     double ex, ey, ez;
     emf3_eval(fx, fy, fz, R, upwind, ex, ey, ez);
@@ -830,7 +835,7 @@ struct EmfPlane {
 };
 template <bool FROM_W>
 KB __global__ void k_emf3_tiled(FSoa FX, FSoa FY, FSoa FZ, Soa W, ERef ref, Ext ce, Ext fe, Box box, int upwind, int iplanes,
-                                double* Ex, double* Ey, double* Ez) {
+                                ET* Ex, ET* Ey, ET* Ez) {
     __shared__ EmfPlane pl[2];
     const int nkt = (box.hi[2] - box.lo[2] + EMF_TK - 1) / EMF_TK;
     const int njt = (box.hi[1] - box.lo[1] + EMF_TJ - 1) / EMF_TJ;
@@ -856,7 +861,7 @@ KB __global__ void k_emf3_tiled(FSoa FX, FSoa FY, FSoa FZ, Soa W, ERef ref, Ext 
                     cross(W(F_BX, q), W(F_BY, q), W(F_BZ, q), W(F_VX, q), W(F_VY, q), W(F_VZ, q), P.r[0][c], P.r[1][c], P.r[2][c]);
                 } else {
                     const long e = fe.at(i, j, k);
-                    P.r[0][c] = ref.x[e]; P.r[1][c] = ref.y[e]; P.r[2][c] = ref.z[e];
+                    P.r[0][c] = ed(ref.x, e); P.r[1][c] = ed(ref.y, e); P.r[2][c] = ed(ref.z, e);
                 }
             }
         }
@@ -892,7 +897,7 @@ KB __global__ void k_emf3_tiled(FSoa FX, FSoa FY, FSoa FZ, Soa W, ERef ref, Ext 
 constexpr int EMF_IPLANES = 32;
 template <bool FROM_W>
 static void emf3_tiled(const FSoa& FX, const FSoa& FY, const FSoa& FZ, const Soa& W, const ERef& ref, const Ext& ce, const Ext& fe,
-                       const Box& eb, int upwind, double* Ex, double* Ey, double* Ez, hipStream_t s) {
+                       const Box& eb, int upwind, ET* Ex, ET* Ey, ET* Ez, hipStream_t s) {
     const long nkt = (eb.hi[2] - eb.lo[2] + EMF_TK - 1) / EMF_TK, njt = (eb.hi[1] - eb.lo[1] + EMF_TJ - 1) / EMF_TJ;
     const long nic = (eb.hi[0] - eb.lo[0] + EMF_IPLANES - 1) / EMF_IPLANES;
     if (nkt <= 0 || njt <= 0 || nic <= 0) return;
@@ -903,7 +908,7 @@ static void emf3_tiled(const FSoa& FX, const FSoa& FY, const FSoa& FZ, const Soa
 // Faraday (B_fields.cpp:21-66) fused with the clone: Bout = Bin everywhere,
 // minus dt*curl(E) on the faces of the reference's ranges (ghost width gg).
 template <int D>
-__device__ __forceinline__ void faraday_body(const double* Ex, const double* Ey, const double* Ez, const double* Bix, const double* Biy,
+__device__ __forceinline__ void faraday_body(const ET* Ex, const ET* Ey, const ET* Ez, const double* Bix, const double* Biy,
                                              const double* Biz, double* Box_, double* Boy, double* Boz, const Ext& fe, int gg, double dt_dx,
                                              double dt_dy, double dt_dz, long m) {
     if (m >= fe.size) return;
@@ -920,18 +925,18 @@ __device__ __forceinline__ void faraday_body(const double* Ex, const double* Ey,
     const bool jy = j >= -gg && j <= ny + gg, jyl = j >= -gg && j < ny + gg;
     const bool kz = k >= -gg && k <= nz + gg, kzl = k >= -gg && k < nz + gg;
     if (D == 2) {
-        if (ix && jyl) bx -= (Ez[m + fe.s[1]] - Ez[m]) * dt_dy;
-        if (ixl && jy) by -= -(Ez[m + fe.s[0]] - Ez[m]) * dt_dx;
-        if (ixl && jyl) bz -= (Ey[m + fe.s[0]] - Ey[m]) * dt_dx - (Ex[m + fe.s[1]] - Ex[m]) * dt_dy;
+        if (ix && jyl) bx -= (ed(Ez, m + fe.s[1]) - ed(Ez, m)) * dt_dy;
+        if (ixl && jy) by -= -(ed(Ez, m + fe.s[0]) - ed(Ez, m)) * dt_dx;
+        if (ixl && jyl) bz -= (ed(Ey, m + fe.s[0]) - ed(Ey, m)) * dt_dx - (ed(Ex, m + fe.s[1]) - ed(Ex, m)) * dt_dy;
     } else {
-        if (ix && jyl && kzl) bx -= (Ez[m + fe.s[1]] - Ez[m]) * dt_dy - (Ey[m + fe.s[2]] - Ey[m]) * dt_dz;
-        if (ixl && jy && kzl) by -= (Ex[m + fe.s[2]] - Ex[m]) * dt_dz - (Ez[m + fe.s[0]] - Ez[m]) * dt_dx;
-        if (ixl && jyl && kz) bz -= (Ey[m + fe.s[0]] - Ey[m]) * dt_dx - (Ex[m + fe.s[1]] - Ex[m]) * dt_dy;
+        if (ix && jyl && kzl) bx -= (ed(Ez, m + fe.s[1]) - ed(Ez, m)) * dt_dy - (ed(Ey, m + fe.s[2]) - ed(Ey, m)) * dt_dz;
+        if (ixl && jy && kzl) by -= (ed(Ex, m + fe.s[2]) - ed(Ex, m)) * dt_dz - (ed(Ez, m + fe.s[0]) - ed(Ez, m)) * dt_dx;
+        if (ixl && jyl && kz) bz -= (ed(Ey, m + fe.s[0]) - ed(Ey, m)) * dt_dx - (ed(Ex, m + fe.s[1]) - ed(Ex, m)) * dt_dy;
     }
     Box_[m] = bx; Boy[m] = by; Boz[m] = bz;
 }
 template <int D>
-KB __global__ void k_faraday(const double* Ex, const double* Ey, const double* Ez, const double* Bix, const double* Biy,
+KB __global__ void k_faraday(const ET* Ex, const ET* Ey, const ET* Ez, const double* Bix, const double* Biy,
                              const double* Biz, double* Box_, double* Boy, double* Boz, Ext fe, int gg, double dt_dx, double dt_dy,
                              double dt_dz) {
     faraday_body<D>(Ex, Ey, Ez, Bix, Biy, Biz, Box_, Boy, Boz, fe, gg, dt_dx, dt_dy, dt_dz, tid());
@@ -969,7 +974,7 @@ __device__ __forceinline__ void cell_B(const double* Bx, const double* By, const
 template <int D>
 __device__ __forceinline__ void whalf_body(const Soa& W, const FSoa& FX, const FSoa& FY, const FSoa& FZ, const double* Bhx, const double* Bhy,
                                            const double* Bhz, const Ext& ce, const Ext& fe, const Box& box, double hx, double hy, double hz,
-                                           const Phys& ph, double* Ehx, double* Ehy, double* Ehz, unsigned long long* err, long t) {
+                                           const Phys& ph, ET* Ehx, ET* Ehy, ET* Ehz, unsigned long long* err, long t) {
     int i, j, k;
     if (!box.point(t, i, j, k)) return;
     const long m = ce.at(i, j, k);
@@ -990,7 +995,7 @@ __device__ __forceinline__ void whalf_body(const Soa& W, const FSoa& FX, const F
 }
 template <int D>
 KB __global__ void k_whalf(Soa W, FSoa FX, FSoa FY, FSoa FZ, const double* Bhx, const double* Bhy, const double* Bhz, Ext ce,
-                           Ext fe, Box box, double hx, double hy, double hz, Phys ph, double* Ehx, double* Ehy, double* Ehz,
+                           Ext fe, Box box, double hx, double hy, double hz, Phys ph, ET* Ehx, ET* Ehy, ET* Ehz,
                            unsigned long long* err) {
     whalf_body<D>(W, FX, FY, FZ, Bhx, Bhy, Bhz, ce, fe, box, hx, hy, hz, ph, Ehx, Ehy, Ehz, err, tid());
 }
@@ -998,7 +1003,7 @@ KB __global__ void k_whalf(Soa W, FSoa FX, FSoa FY, FSoa FZ, const double* Bhx, 
 // ctu_sweep_MHD corrections (CTU.cpp:65-129 3D, 162-202 2D), in place.
 template <int D>
 KB __global__ void k_ctu_mhd(Soa W, const double* Bx, const double* By, const double* Bz, const double* Bhx, const double* Bhy,
-                             const double* Bhz, const double* E0x, const double* E0y, const double* E0z, FSoa FX, FSoa FY, FSoa FZ,
+                             const double* Bhz, const ET* E0x, const ET* E0y, const ET* E0z, FSoa FX, FSoa FY, FSoa FZ,
                              Soa XL, Soa XR, Soa YL, Soa YR, Soa ZL, Soa ZR, Ext ce, Ext fe, Box box, double dt_dx, double dt_dy,
                              double dt_dz, Phys ph) {
     int i, j, k;
@@ -1024,13 +1029,13 @@ KB __global__ void k_ctu_mhd(Soa W, const double* Bx, const double* By, const do
         corr.Bx = corr.By = corr.Bz = 0;
         double lim_z = minmod(-dBy, dBx);
         corr.E += k8 * (Wc.Bz * Wc.vz * lim_z);
-        corr.Bz = 0.5 * dt_dy * (E0x[q + fy] - E0x[q]) + 0.5 * Wc.vz * lim_z;
+        corr.Bz = 0.5 * dt_dy * (ed(E0x, q + fy) - ed(E0x, q)) + 0.5 * Wc.vz * lim_z;
         apply(XL, XR, corr, 0);
         corr = umul<true>(umul<true>(fd(FX, 0), 0.5), dt_dx);
         corr.Bx = corr.By = corr.Bz = 0;
         lim_z = minmod(-dBx, dBy);
         corr.E += k8 * (Wc.Bz * Wc.vz * lim_z);
-        corr.Bz = -0.5 * dt_dx * (E0y[q + fx] - E0y[q]) + 0.5 * Wc.vz * lim_z;
+        corr.Bz = -0.5 * dt_dx * (ed(E0y, q + fx) - ed(E0y, q)) + 0.5 * Wc.vz * lim_z;
         apply(YL, YR, corr, 1);
     } else {
         const double dBz = (Bz[q + fz] - Bz[q]) * dt_dz;
@@ -1039,8 +1044,8 @@ KB __global__ void k_ctu_mhd(Soa W, const double* Bx, const double* By, const do
         corr.Bx = corr.By = corr.Bz = 0;
         double lim_y = minmod(-dBz, dBx), lim_z = minmod(-dBy, dBx);
         corr.E += k8 * (Wc.By * Wc.vy * lim_y + Wc.Bz * Wc.vz * lim_z);
-        corr.By = -0.25 * dt_dz * (E0x[q + fz] - E0x[q] + E0x[q + fy + fz] - E0x[q + fy]) + 0.5 * Wc.vy * lim_y;
-        corr.Bz = 0.25 * dt_dy * (E0x[q + fy] - E0x[q] + E0x[q + fy + fz] - E0x[q + fz]) + 0.5 * Wc.vz * lim_z;
+        corr.By = -0.25 * dt_dz * (ed(E0x, q + fz) - ed(E0x, q) + ed(E0x, q + fy + fz) - ed(E0x, q + fy)) + 0.5 * Wc.vy * lim_y;
+        corr.Bz = 0.25 * dt_dy * (ed(E0x, q + fy) - ed(E0x, q) + ed(E0x, q + fy + fz) - ed(E0x, q + fz)) + 0.5 * Wc.vz * lim_z;
         apply(XL, XR, corr, 0);
         // y half states
         corr = umul<true>(uadd<true>(umul<true>(fd(FX, 0), dt_dx), umul<true>(fd(FZ, 2), dt_dz)), 0.5);
@@ -1048,8 +1053,8 @@ KB __global__ void k_ctu_mhd(Soa W, const double* Bx, const double* By, const do
         double lim_x = minmod(-dBz, dBy);
         lim_z = minmod(-dBx, dBy);
         corr.E += k8 * (Wc.Bx * Wc.vx * lim_x + Wc.Bz * Wc.vz * lim_z);
-        corr.Bx = 0.25 * dt_dz * (E0y[q + fz] - E0y[q] + E0y[q + fx + fz] - E0y[q + fx]) + 0.5 * Wc.vx * lim_x;
-        corr.Bz = -0.25 * dt_dx * (E0y[q + fx] - E0y[q] + E0y[q + fx + fz] - E0y[q + fz]) + 0.5 * Wc.vz * lim_z;
+        corr.Bx = 0.25 * dt_dz * (ed(E0y, q + fz) - ed(E0y, q) + ed(E0y, q + fx + fz) - ed(E0y, q + fx)) + 0.5 * Wc.vx * lim_x;
+        corr.Bz = -0.25 * dt_dx * (ed(E0y, q + fx) - ed(E0y, q) + ed(E0y, q + fx + fz) - ed(E0y, q + fz)) + 0.5 * Wc.vz * lim_z;
         apply(YL, YR, corr, 1);
         // z half states
         corr = umul<true>(uadd<true>(umul<true>(fd(FX, 0), dt_dx), umul<true>(fd(FY, 1), dt_dy)), 0.5);
@@ -1057,8 +1062,8 @@ KB __global__ void k_ctu_mhd(Soa W, const double* Bx, const double* By, const do
         lim_x = minmod(-dBy, dBz);
         lim_y = minmod(-dBx, dBz);
         corr.E += k8 * (Wc.Bx * Wc.vx * lim_x + Wc.By * Wc.vy * lim_y);
-        corr.Bx = -0.25 * dt_dy * (E0z[q + fy] - E0z[q] + E0z[q + fx + fy] - E0z[q + fx]) + 0.5 * Wc.vx * lim_x;
-        corr.By = 0.25 * dt_dx * (E0z[q + fx] - E0z[q] + E0z[q + fx + fy] - E0z[q + fy]) + 0.5 * Wc.vy * lim_y;
+        corr.Bx = -0.25 * dt_dy * (ed(E0z, q + fy) - ed(E0z, q) + ed(E0z, q + fx + fy) - ed(E0z, q + fx)) + 0.5 * Wc.vx * lim_x;
+        corr.By = 0.25 * dt_dx * (ed(E0z, q + fx) - ed(E0z, q) + ed(E0z, q + fx + fy) - ed(E0z, q + fy)) + 0.5 * Wc.vy * lim_y;
         apply(ZL, ZR, corr, 2);
     }
 }
@@ -1276,7 +1281,7 @@ static void sweep_t(const StepCtx& c, const Box& b, const FSoa& out, int stage, 
     SweepArgs a{};
     a.W = c.W;
     for (int q = 0; q < 3; q++) {
-        a.B[q] = c.B[q]; a.Bh[q] = c.Bh[q]; a.E0[q] = c.E0[q]; a.Fp[q] = fs(c.Fp[q]);
+        a.B[q] = c.B[q]; a.Bh[q] = c.Bh[q]; a.E0[q] = et(c.E0[q]); a.Fp[q] = fs(c.Fp[q]);
         a.lo[q] = b.lo[q]; a.hi[q] = b.hi[q];
         a.dt_d[q] = c.dt / c.d[q];
     }
@@ -1337,17 +1342,17 @@ static void step_fused(const StepCtx& c, hipStream_t s) {
             Box eb = make_box(D, -1, ce.n[0] + 2, -1, ce.n[1] + 2, -1, ce.n[2] + 2);
             if constexpr (D == 2)
                 hipLaunchKernelGGL((k_emf2<true>), dim3(blocks(eb.count())), dim3(256), 0, s, fs(c.Fp[0]), fs(c.Fp[1]), c.W, none, ce, c.fe, eb, 1,
-                                   c.E0[0], c.E0[1], c.E0[2]);
+                                   et(c.E0[0]), et(c.E0[1]), et(c.E0[2]));
             else
-                emf3_tiled<true>(fs(c.Fp[0]), fs(c.Fp[1]), fs(c.Fp[2]), c.W, none, ce, c.fe, eb, 1, c.E0[0], c.E0[1], c.E0[2], s);
+                emf3_tiled<true>(fs(c.Fp[0]), fs(c.Fp[1]), fs(c.Fp[2]), c.W, none, ce, c.fe, eb, 1, et(c.E0[0]), et(c.E0[1]), et(c.E0[2]), s);
             mark(c, K_EMF_PRED);
-            hipLaunchKernelGGL((k_faraday<D>), dim3(blocks(c.fe.size)), dim3(256), 0, s, c.E0[0], c.E0[1], c.E0[2], c.B[0], c.B[1],
+            hipLaunchKernelGGL((k_faraday<D>), dim3(blocks(c.fe.size)), dim3(256), 0, s, et(c.E0[0]), et(c.E0[1]), et(c.E0[2]), c.B[0], c.B[1],
                                c.B[2], c.Bh[0], c.Bh[1], c.Bh[2], c.fe, 1, 0.5 * dt_d[0], 0.5 * dt_d[1], 0.5 * dt_d[2]);
             mark(c, K_FARADAY_HALF);
             Box cb = make_box(D, -1, ce.n[0] + 1, -1, ce.n[1] + 1, -1, ce.n[2] + 1);
             hipLaunchKernelGGL((k_whalf<D>), dim3(blocks(cb.count())), dim3(256), 0, s, c.W, fs(c.Fp[0]), fs(c.Fp[1]), fs(c.Fp[2]), c.Bh[0],
-                               c.Bh[1], c.Bh[2], ce, c.fe, cb, 0.5 * dt_d[0], 0.5 * dt_d[1], 0.5 * dt_d[2], c.ph, c.Eh[0], c.Eh[1],
-                               c.Eh[2], c.err);
+                               c.Bh[1], c.Bh[2], ce, c.fe, cb, 0.5 * dt_d[0], 0.5 * dt_d[1], 0.5 * dt_d[2], c.ph, et(c.Eh[0]), et(c.Eh[1]),
+                               et(c.Eh[2]), c.err);
             mark(c, K_WHALF);
         }
     }
@@ -1359,17 +1364,17 @@ static void step_fused(const StepCtx& c, hipStream_t s) {
     }
     if constexpr (MHD) {
         Box eb = make_box(D, 0, ce.n[0] + 1, 0, ce.n[1] + 1, 0, ce.n[2] + 1);
-        ERef eh{c.Eh[0], c.Eh[1], c.Eh[2]};
+        ERef eh{et(c.Eh[0]), et(c.Eh[1]), et(c.Eh[2])};
         if constexpr (D == 2)
             hipLaunchKernelGGL((k_emf2<false>), dim3(blocks(eb.count())), dim3(256), 0, s, fs(c.F[0]), fs(c.F[1]), c.W, eh, ce, c.fe, eb, c.ctu,
-                               c.E0[0], c.E0[1], c.E0[2]);
+                               et(c.E0[0]), et(c.E0[1]), et(c.E0[2]));
         else if constexpr (sizeof(FT) == 8)  // FP64 fluxes: the LDS planes allow 2 workgroups per CU, slower than k_emf3 here
             hipLaunchKernelGGL((k_emf3<false>), dim3(blocks(eb.count())), dim3(256), 0, s, fs(c.F[0]), fs(c.F[1]), fs(c.F[2]), c.W, eh, ce, c.fe,
-                               eb, c.ctu, c.E0[0], c.E0[1], c.E0[2]);
+                               eb, c.ctu, et(c.E0[0]), et(c.E0[1]), et(c.E0[2]));
         else
-            emf3_tiled<false>(fs(c.F[0]), fs(c.F[1]), fs(c.F[2]), c.W, eh, ce, c.fe, eb, c.ctu, c.E0[0], c.E0[1], c.E0[2], s);
+            emf3_tiled<false>(fs(c.F[0]), fs(c.F[1]), fs(c.F[2]), c.W, eh, ce, c.fe, eb, c.ctu, et(c.E0[0]), et(c.E0[1]), et(c.E0[2]), s);
         mark(c, K_EMF);
-        hipLaunchKernelGGL((k_faraday<D>), dim3(blocks(c.fe.size)), dim3(256), 0, s, c.E0[0], c.E0[1], c.E0[2], c.B[0], c.B[1], c.B[2],
+        hipLaunchKernelGGL((k_faraday<D>), dim3(blocks(c.fe.size)), dim3(256), 0, s, et(c.E0[0]), et(c.E0[1]), et(c.E0[2]), c.B[0], c.B[1], c.B[2],
                            c.Bn[0], c.Bn[1], c.Bn[2], c.fe, 0, dt_d[0], dt_d[1], dt_d[2]);
         mark(c, K_FARADAY);
     }
@@ -1417,21 +1422,21 @@ static void step_t(const StepCtx& c, hipStream_t s) {
             Box eb = make_box(D, -1, ce.n[0] + 2, -1, ce.n[1] + 2, -1, ce.n[2] + 2);
             if constexpr (D == 2)
                 hipLaunchKernelGGL((k_emf2<true>), dim3(blocks(eb.count())), dim3(256), 0, s, fs(c.F[0]), fs(c.F[1]), c.W, none, ce, c.fe, eb, 1,
-                                   c.E0[0], c.E0[1], c.E0[2]);
+                                   et(c.E0[0]), et(c.E0[1]), et(c.E0[2]));
             else
                 hipLaunchKernelGGL((k_emf3<true>), dim3(blocks(eb.count())), dim3(256), 0, s, fs(c.F[0]), fs(c.F[1]), fs(c.F[2]), c.W, none, ce, c.fe,
-                                   eb, 1, c.E0[0], c.E0[1], c.E0[2]);
+                                   eb, 1, et(c.E0[0]), et(c.E0[1]), et(c.E0[2]));
             mark(c, K_EMF_PRED);
-            hipLaunchKernelGGL((k_faraday<D>), dim3(blocks(c.fe.size)), dim3(256), 0, s, c.E0[0], c.E0[1], c.E0[2], c.B[0], c.B[1],
+            hipLaunchKernelGGL((k_faraday<D>), dim3(blocks(c.fe.size)), dim3(256), 0, s, et(c.E0[0]), et(c.E0[1]), et(c.E0[2]), c.B[0], c.B[1],
                                c.B[2], c.Bh[0], c.Bh[1], c.Bh[2], c.fe, 1, 0.5 * dt_d[0], 0.5 * dt_d[1], 0.5 * dt_d[2]);
             mark(c, K_FARADAY_HALF);
             Box cb = make_box(D, -1, ce.n[0] + 1, -1, ce.n[1] + 1, -1, ce.n[2] + 1);
             hipLaunchKernelGGL((k_whalf<D>), dim3(blocks(cb.count())), dim3(256), 0, s, c.W, fs(c.F[0]), fs(c.F[1]), fs(c.F[2]), c.Bh[0], c.Bh[1],
-                               c.Bh[2], ce, c.fe, cb, 0.5 * dt_d[0], 0.5 * dt_d[1], 0.5 * dt_d[2], c.ph, c.Eh[0], c.Eh[1], c.Eh[2],
+                               c.Bh[2], ce, c.fe, cb, 0.5 * dt_d[0], 0.5 * dt_d[1], 0.5 * dt_d[2], c.ph, et(c.Eh[0]), et(c.Eh[1]), et(c.Eh[2]),
                                c.err);
             mark(c, K_WHALF);
             hipLaunchKernelGGL((k_ctu_mhd<D>), dim3(blocks(cb.count())), dim3(256), 0, s, c.W, c.B[0], c.B[1], c.B[2], c.Bh[0], c.Bh[1],
-                               c.Bh[2], c.E0[0], c.E0[1], c.E0[2], fs(c.F[0]), fs(c.F[1]), fs(c.F[2]), c.HS[0], c.HS[1], c.HS[2], c.HS[3],
+                               c.Bh[2], et(c.E0[0]), et(c.E0[1]), et(c.E0[2]), fs(c.F[0]), fs(c.F[1]), fs(c.F[2]), c.HS[0], c.HS[1], c.HS[2], c.HS[3],
                                c.HS[4], c.HS[5], ce, c.fe, cb, dt_d[0], dt_d[1], dt_d[2], c.ph);
             mark(c, K_CTU);
         } else if constexpr (D == 2) {
@@ -1469,15 +1474,15 @@ static void step_t(const StepCtx& c, hipStream_t s) {
     }
     if constexpr (MHD) {
         Box eb = make_box(D, 0, ce.n[0] + 1, 0, ce.n[1] + 1, 0, ce.n[2] + 1);
-        ERef eh{c.Eh[0], c.Eh[1], c.Eh[2]};
+        ERef eh{et(c.Eh[0]), et(c.Eh[1]), et(c.Eh[2])};
         if constexpr (D == 2)
             hipLaunchKernelGGL((k_emf2<false>), dim3(blocks(eb.count())), dim3(256), 0, s, fs(c.F[0]), fs(c.F[1]), c.W, eh, ce, c.fe, eb, c.ctu,
-                               c.E0[0], c.E0[1], c.E0[2]);
+                               et(c.E0[0]), et(c.E0[1]), et(c.E0[2]));
         else
             hipLaunchKernelGGL((k_emf3<false>), dim3(blocks(eb.count())), dim3(256), 0, s, fs(c.F[0]), fs(c.F[1]), fs(c.F[2]), c.W, eh, ce, c.fe, eb,
-                               c.ctu, c.E0[0], c.E0[1], c.E0[2]);
+                               c.ctu, et(c.E0[0]), et(c.E0[1]), et(c.E0[2]));
         mark(c, K_EMF);
-        hipLaunchKernelGGL((k_faraday<D>), dim3(blocks(c.fe.size)), dim3(256), 0, s, c.E0[0], c.E0[1], c.E0[2], c.B[0], c.B[1], c.B[2],
+        hipLaunchKernelGGL((k_faraday<D>), dim3(blocks(c.fe.size)), dim3(256), 0, s, et(c.E0[0]), et(c.E0[1]), et(c.E0[2]), c.B[0], c.B[1], c.B[2],
                            c.Bn[0], c.Bn[1], c.Bn[2], c.fe, 0, dt_d[0], dt_d[1], dt_d[2]);
         mark(c, K_FARADAY);
     }
@@ -1546,7 +1551,8 @@ void ctu_hydro3(const StepCtx& c, hipStream_t s) {
 struct PatchView {
     Ext ce, fe;
     Soa W, Wn;
-    double *B[3], *Bn[3], *Bh[3], *E0[3], *Eh[3];
+    double *B[3], *Bn[3], *Bh[3];
+    ET *E0[3], *Eh[3];
     FSoa Fp[3], F[3];
     double dx, dy;
     int cls;
@@ -1564,8 +1570,8 @@ __device__ __forceinline__ PatchView patch_view(const BatchArgs& A, int p) {
         v.B[c] = A.B[par] + 3 * P.of + c * P.nfc;
         v.Bn[c] = A.B[1 - par] + 3 * P.of + c * P.nfc;
         v.Bh[c] = A.Bh + 3 * P.of + c * P.nfc;
-        v.E0[c] = A.E0 + 3 * P.of + c * P.nfc;
-        v.Eh[c] = A.Eh + 3 * P.of + c * P.nfc;
+        v.E0[c] = et(A.E0) + 3 * P.of + c * P.nfc;  // EMF pools hold ET elements at the face offsets
+        v.Eh[c] = et(A.Eh) + 3 * P.of + c * P.nfc;
     }
     for (int ax = 0; ax < 2; ax++) {  // flux pools hold FT elements at the same per-patch offsets
         v.Fp[ax] = FSoa{reinterpret_cast<FT*>(A.Fp) + 16 * P.oc + ax * 8 * P.nc, P.nc};
