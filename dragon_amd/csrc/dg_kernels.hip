@@ -658,8 +658,12 @@ __device__ __forceinline__ void sweep_wave_body(const SweepArgs& a, long wave, i
     const P<R> Rl = shfl_up1(Rr);
     if (lane >= 1 && live) face_flux<MHD, R, AX>(a, Rl, L, a.ce.at(idx[0], idx[1], idx[2]));
 }
+// Occupancy floor of the contiguous-axis sweeps: the FP64 ones need 176
+// registers, 8 short of three waves per SIMD; at three they spill 12 bytes
+// per lane.  1 = the compiler's choice.
+template <class R> constexpr int wave_min_waves() { return DG_SWEEP_WAVES == 0 && sizeof(R) == 8 ? 3 : 1; }
 template <int D, bool MHD, class R, int AX, bool FINAL>
-KS __global__ void k_sweep_wave(SweepArgs a) {
+KS __attribute__((amdgpu_waves_per_eu(wave_min_waves<R>()))) __global__ void k_sweep_wave(SweepArgs a) {
     // the wavefront index is uniform: read it into a scalar register so the
     // index arithmetic of sweep_wave_body runs once per wavefront on the SALU
     const int w = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
@@ -1694,7 +1698,7 @@ KS __global__ void kb_sweep_x(BatchArgs A, BlockTable bt) {
     sweep_march_body<2, true, RT, 0, FIN>(s, t);
 }
 template <bool FIN>
-KS __global__ void kb_sweep_y(BatchArgs A, BlockTable bt) {
+KS __attribute__((amdgpu_waves_per_eu(wave_min_waves<RT>()))) __global__ void kb_sweep_y(BatchArgs A, BlockTable bt) {
     BATCH_ITEM(bt)  // first is a multiple of the workgroup size: first / 64 + the wavefront in the workgroup
     const PatchView v = patch_view(A, p);
     SweepArgs s = batch_sweep_args<1, FIN>(A, v, p);
