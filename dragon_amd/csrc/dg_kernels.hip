@@ -408,6 +408,43 @@ struct SweepArgs {
     int stage, chunk;
 };
 
+// Conversions of the CTU-corrected half states.  The FP64 build keeps the
+// reference's divisions.  The mixed build narrows these states to FP32 right
+// after, so it divides through a refined reciprocal (v_rcp_f64 and two
+// Newton steps, within an ulp of the quotient) in place of the correctly
+// rounded FP64 division sequence; the energy difference of the pressure
+// recovery is still formed in FP64.
+#if DG_PREC == 32
+__device__ __forceinline__ double rcp64(double x) {
+    double r = __builtin_amdgcn_rcp(x);
+    r = fma(fma(-x, r, 1.0), r, r);
+    return fma(fma(-x, r, 1.0), r, r);
+}
+template <bool MHD> __device__ __forceinline__ U<double> cons_c(const P<double>& w, const Phys& ph) {
+    U<double> u;
+    u.rho = w.rho;
+    u.mx = w.vx * w.rho; u.my = w.vy * w.rho; u.mz = w.vz * w.rho;
+    u.E = w.p * (0.5 * ph._2_Gm1) + (w.rho / 2) * dot3(w.vx, w.vy, w.vz, w.vx, w.vy, w.vz);
+    if constexpr (MHD) u.E += dot3(w.Bx, w.By, w.Bz, w.Bx, w.By, w.Bz) * ph.k1_8pi;
+    if constexpr (MHD) { u.Bx = w.Bx; u.By = w.By; u.Bz = w.Bz; } else { u.Bx = u.By = u.Bz = 0; }
+    return u;
+}
+template <bool MHD> __device__ __forceinline__ P<double> prim_c(const U<double>& u, const Phys& ph) {
+    const double inv = rcp64(u.rho);
+    P<double> w;
+    w.rho = u.rho;
+    w.vx = u.mx * inv; w.vy = u.my * inv; w.vz = u.mz * inv;
+    double Et = u.E - dot3(u.mx, u.my, u.mz, u.mx, u.my, u.mz) * (0.5 * inv);
+    if constexpr (MHD) Et -= dot3(u.Bx, u.By, u.Bz, u.Bx, u.By, u.Bz) * ph.k1_8pi;
+    w.p = ph.gm1 * Et;
+    if constexpr (MHD) { w.Bx = u.Bx; w.By = u.By; w.Bz = u.Bz; } else { w.Bx = w.By = w.Bz = 0; }
+    return w;
+}
+#else
+template <bool MHD> __device__ __forceinline__ U<double> cons_c(const P<double>& w, const Phys& ph) { return cons<MHD>(w, ph); }
+template <bool MHD> __device__ __forceinline__ P<double> prim_c(const U<double>& u, const Phys& ph) { return prim<MHD>(u, ph); }
+#endif
+
 // CTU correction of the half states of one cell (double precision).
 template <int D, bool MHD, int AX>
 __device__ __forceinline__ void ctu_correct(const SweepArgs& a, long m, long q, double dBx, double dBy, double dBz,
@@ -418,8 +455,8 @@ __device__ __forceinline__ void ctu_correct(const SweepArgs& a, long m, long q, 
     if constexpr (!MHD) {  // correctState (hydro 2D): minus the transverse flux difference
         constexpr int T = AX == 0 ? 1 : 0;
         U<double> t = umul<false>(usub<false>(load_u(a.Fp[T], m + ce.s[T], false), load_u(a.Fp[T], m, false)), 0.5 * a.dt_d[T]);
-        L = prim<false>(usub<false>(cons<false>(L, ph), t), ph);
-        Rr = prim<false>(usub<false>(cons<false>(Rr, ph), t), ph);
+        L = prim_c<false>(usub<false>(cons_c<false>(L, ph), t), ph);
+        Rr = prim_c<false>(usub<false>(cons_c<false>(Rr, ph), t), ph);
         return;
     } else {
         const Ext& fe = a.fe;
@@ -465,14 +502,14 @@ __device__ __forceinline__ void ctu_correct(const SweepArgs& a, long m, long q, 
             corr.Bx = -0.25 * dt_dy * (ed(E0z, q + fy) - ed(E0z, q) + ed(E0z, q + fx + fy) - ed(E0z, q + fx)) + 0.5 * Wc.vx * lim_x;
             corr.By = 0.25 * dt_dx * (ed(E0z, q + fx) - ed(E0z, q) + ed(E0z, q + fx + fy) - ed(E0z, q + fy)) + 0.5 * Wc.vy * lim_y;
         }
-        U<double> uL = uadd<true>(cons<true>(L, ph), corr), uR = uadd<true>(cons<true>(Rr, ph), corr);
+        U<double> uL = uadd<true>(cons_c<true>(L, ph), corr), uR = uadd<true>(cons_c<true>(Rr, ph), corr);
         const double* Bh = a.Bh[AX];
         const long e = fe.s[AX];
         if (AX == 0) { uL.Bx = Bh[q]; uR.Bx = Bh[q + e]; }
         else if (AX == 1) { uL.By = Bh[q]; uR.By = Bh[q + e]; }
         else { uL.Bz = Bh[q]; uR.Bz = Bh[q + e]; }
-        L = prim<true>(uL, ph);
-        Rr = prim<true>(uR, ph);
+        L = prim_c<true>(uL, ph);
+        Rr = prim_c<true>(uR, ph);
     }
 }
 
