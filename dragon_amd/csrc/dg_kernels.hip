@@ -103,29 +103,27 @@ __device__ void copy_cell(const Soa& W, long dst, long src, int nf) {
     for (int f = 0; f < nf; f++) W(f, dst) = W(f, src);
 }
 
-__device__ __forceinline__ void bc_cells_body(const BCArgs& a, long t) {
-    const int n1 = a.hi[0] - a.lo[0], n2 = a.hi[1] - a.lo[1];
-    if (t >= (long)n1 * n2) return;
-    const int t0 = a.lo[0] + (int)(t / n2), t1 = a.lo[1] + (int)(t % n2);
+// Ghost cell g (1..ng) on side `side` of the line (t0, t1).
+__device__ __forceinline__ void bc_cell(const BCArgs& a, int t0, int t1, int g, int side) {
     const Ext& e = a.ce;
-    const int n = e.n[a.axis], ng = e.g;
+    const int n = e.n[a.axis];
     const int nf = a.nf;
     const int vf = F_VX + a.axis, bfn = F_BX + a.axis;
-    for (int g = 1; g <= ng; g++) {
-        if (a.kind == 1) {  // Periodic (Periodic.cpp): both sides per g
-            copy_cell(a.W, cidx(e, a.axis, -g, t0, t1), cidx(e, a.axis, n - g, t0, t1), nf);
-            copy_cell(a.W, cidx(e, a.axis, n - 1 + g, t0, t1), cidx(e, a.axis, g - 1, t0, t1), nf);
-            continue;
+    {
+        if (a.kind == 1) {  // Periodic (Periodic.cpp)
+            if (side == 0) copy_cell(a.W, cidx(e, a.axis, -g, t0, t1), cidx(e, a.axis, n - g, t0, t1), nf);
+            else copy_cell(a.W, cidx(e, a.axis, n - 1 + g, t0, t1), cidx(e, a.axis, g - 1, t0, t1), nf);
+            return;
         }
-        const long dst = cidx(e, a.axis, a.side == 0 ? -g : n - 1 + g, t0, t1);
+        const long dst = cidx(e, a.axis, side == 0 ? -g : n - 1 + g, t0, t1);
         if (a.kind == 0) {  // Outflow (+gated) (Outflow.cpp)
-            copy_cell(a.W, dst, cidx(e, a.axis, a.side == 0 ? 0 : n - 1, t0, t1), nf);
+            copy_cell(a.W, dst, cidx(e, a.axis, side == 0 ? 0 : n - 1, t0, t1), nf);
             if (a.gated) {
                 double v = a.W(vf, dst);
-                if (a.side == 0 ? v > 0 : v < 0) a.W(vf, dst) = 0;
+                if (side == 0 ? v > 0 : v < 0) a.W(vf, dst) = 0;
             }
         } else if (a.kind == 2) {  // Reflective (Reflective.cpp)
-            copy_cell(a.W, dst, cidx(e, a.axis, a.side == 0 ? g - 1 : n - g, t0, t1), nf);
+            copy_cell(a.W, dst, cidx(e, a.axis, side == 0 ? g - 1 : n - g, t0, t1), nf);
             a.W(vf, dst) *= -1;
             if (nf == 8) {
                 for (int f = F_BX; f <= F_BZ; f++) a.W(f, dst) *= -1;
@@ -136,7 +134,33 @@ __device__ __forceinline__ void bc_cells_body(const BCArgs& a, long t) {
         }
     }
 }
+// One thread per line, the reference's g loop in order (periodic: both
+// sides per g).
+__device__ __forceinline__ void bc_cells_body(const BCArgs& a, long t) {
+    const int n1 = a.hi[0] - a.lo[0], n2 = a.hi[1] - a.lo[1];
+    if (t >= (long)n1 * n2) return;
+    const int t0 = a.lo[0] + (int)(t / n2), t1 = a.lo[1] + (int)(t % n2);
+    for (int g = 1; g <= a.ce.g; g++) {
+        if (a.kind == 1) { bc_cell(a, t0, t1, g, 0); bc_cell(a, t0, t1, g, 1); }
+        else bc_cell(a, t0, t1, g, a.side);
+    }
+}
 KB __global__ void k_bc_cells(BCArgs a) { bc_cells_body(a, tid()); }
+// The same with one thread per ghost cell, for the contiguous axis: the
+// cells a pass writes are distinct and, when the axis holds at least ng
+// cells, every cell it reads is interior, so the g loop has no order to
+// keep.  Neighbouring lanes then write neighbouring ghost cells of a line
+// instead of one cell per line each.
+KB __global__ void k_bc_cells_g(BCArgs a) {
+    const int ng = a.ce.g, per = a.kind == 1 ? 2 * ng : ng;
+    const int n1 = a.hi[0] - a.lo[0], n2 = a.hi[1] - a.lo[1];
+    long t = tid();
+    if (t >= (long)n1 * n2 * per) return;
+    const int slot = (int)(t % per);
+    t /= per;
+    const int t0 = a.lo[0] + (int)(t / n2), t1 = a.lo[1] + (int)(t % n2);
+    bc_cell(a, t0, t1, slot % ng + 1, a.kind == 1 ? slot / ng : a.side);
+}
 
 __device__ __forceinline__ void fget(double* const B[3], long m, double v[3]) { v[0] = B[0][m]; v[1] = B[1][m]; v[2] = B[2][m]; }
 __device__ __forceinline__ void fset(double* const B[3], long m, const double v[3]) { B[0][m] = v[0]; B[1][m] = v[1]; B[2][m] = v[2]; }
@@ -201,6 +225,22 @@ __device__ __forceinline__ void bc_faces_body(const BCArgs& a, long t) {
     }
 }
 KB __global__ void k_bc_faces(BCArgs a) { bc_faces_body(a, tid()); }
+// Periodic face pass on the contiguous axis, one thread per ghost face:
+// with at least ng cells along the axis every face it reads is interior.
+KB __global__ void k_bc_faces_periodic_g(BCArgs a) {
+    const int ng = a.fe.g, per = 2 * ng;
+    const int n1 = a.hi[0] - a.lo[0], n2 = a.hi[1] - a.lo[1];
+    long t = tid();
+    if (t >= (long)n1 * n2 * per) return;
+    const int slot = (int)(t % per);
+    t /= per;
+    const int t0 = a.lo[0] + (int)(t / n2), t1 = a.lo[1] + (int)(t % n2);
+    const int g = slot % ng + 1, n = a.ce.n[a.axis];
+    auto at = [&](int q) { return cidx(a.fe, a.axis, q, t0, t1); };
+    double v[3];
+    if (slot < ng) { fget(a.B, at(n - g), v); fset(a.B, at(-g), v); }
+    else { fget(a.B, at(g - 1), v); fset(a.B, at(n - 1 + g), v); }
+}
 
 // ==================================================================== CFL
 // Max signal speed over cells [-1, n+1)^D (CFL.cpp:113-162); wave64
@@ -1255,7 +1295,12 @@ void launch_divb(int dim, double* B, Ext ce, Ext fe, const double d[3], unsigned
 void launch_bc(const BCArgs& a, bool faces, hipStream_t s) {
     long n = (long)(a.hi[0] - a.lo[0]) * (a.hi[1] - a.lo[1]);
     if (n <= 0) return;
-    if (faces) hipLaunchKernelGGL(k_bc_faces, dim3(blocks(n)), dim3(256), 0, s, a);
+    const bool contiguous = a.axis == a.dim - 1 && a.dim >= 2 && a.ce.n[a.axis] >= a.ce.g;
+    if (faces && contiguous && a.kind == 1)
+        hipLaunchKernelGGL(k_bc_faces_periodic_g, dim3(blocks(n * 2 * a.fe.g)), dim3(256), 0, s, a);
+    else if (faces) hipLaunchKernelGGL(k_bc_faces, dim3(blocks(n)), dim3(256), 0, s, a);
+    else if (contiguous)
+        hipLaunchKernelGGL(k_bc_cells_g, dim3(blocks(n * (a.kind == 1 ? 2 : 1) * a.ce.g)), dim3(256), 0, s, a);
     else hipLaunchKernelGGL(k_bc_cells, dim3(blocks(n)), dim3(256), 0, s, a);
 }
 
