@@ -71,10 +71,9 @@ def test_fp64_parity_100_steps_2d_mhd():
     run_pair(P.orszag_tang_2d(48), steps=100)
 
 
-def rect_mhd_3d():
-    """Non-cubic 3D MHD (CTU + HLLD + CT, van Leer) on 40 x 9 x 70 cells with a
-    random state."""
-    shape = (40, 9, 70)
+def rect_mhd_3d(shape=(40, 9, 70)):
+    """Non-cubic 3D MHD (CTU + HLLD + CT, van Leer, periodic) with a random
+    state, 40 x 9 x 70 cells by default."""
     g = P.ghosts_for(True)
     w, B = P.empty_state(shape, g, True)
     rng = np.random.default_rng(7)
@@ -84,13 +83,23 @@ def rect_mhd_3d():
     wi[..., 2:5] = 0.5 * (rng.random(shape + (3,)) - 0.5)
     P.interior(B, g, shape, extra=1)[..., 0] = 0.2
     cfg = dict(mhd=1, gamma=5.0 / 3.0, solver=5, limiter=2, ctu=1, ct_energy=2)
-    return (shape, (1 / 40, 1 / 9, 1 / 70), cfg, [(1, 63)], w, B, {"init_B": True})
+    return (shape, tuple(1.0 / n for n in shape), cfg, [(1, 63)], w, B, {"init_B": True})
 
 
 def test_mhd_3d_rectangular():
     """rect_mhd_3d: several i-runs, j-tiles and k-tiles of the blocked EMF pass,
     partial tiles on every side."""
     run_pair(rect_mhd_3d(), steps=5)
+
+
+@pytest.mark.parametrize("nz", [2, 3])
+def test_thin_contiguous_axis(nz):
+    """Ghost passes on the contiguous axis run one thread per ghost cell when
+    the axis holds at least ng = 3 cells (every cell they read is then
+    interior); below that the reference's in-order g loop, whose periodic
+    images read ghosts written earlier in the same pass.  Both sides of the
+    switch against the oracle."""
+    run_pair(rect_mhd_3d((10, 12, nz)), steps=3)
 
 
 @pytest.mark.parametrize("case,chunk", [("c2", 7), ("c3", 7), ("c4", 5), ("rect", 7), ("rect", 64)])
