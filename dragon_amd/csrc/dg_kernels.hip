@@ -685,11 +685,23 @@ __device__ __forceinline__ void sweep_march_body(const SweepArgs& a, long t) {
             Rp = Rc;
         }
     } else {
-        cell_states<D, MHD, R, AX, FINAL>(a, idx[0], idx[1], idx[2], Lp, Rp);
+        // The cell ahead of the stencil is read before the Riemann solve of
+        // the current face; the two behind it are read again (cache hits):
+        // holding them too takes the FP64 walker to 256 registers.
+        const long s = a.ce.s[AX];
+        const int clo = -a.ce.g, chi = a.ce.n[AX] + a.ce.g;  // cells that exist along AX
+        long m = a.ce.at(idx[0], idx[1], idx[2]);
+        auto ld = [&](int c, long mm) { return c >= clo && c < chi ? load_p<double>(a.W, mm, MHD) : pzero<double>(); };
+        cell_states_from<D, MHD, R, AX, FINAL>(a, idx[0], idx[1], idx[2], p_cast<R>(ld(f0 - 2, m - s)), p_cast<R>(ld(f0 - 1, m)),
+                                               p_cast<R>(ld(f0, m + s)), Lp, Rp);
+        P<double> nxt = ld(f0 + 1, m + 2 * s);
         for (int f = f0; f < f1; f++) {
             idx[AX] = f;
-            cell_states<D, MHD, R, AX, FINAL>(a, idx[0], idx[1], idx[2], Lc, Rc);
-            face_flux<MHD, R, AX>(a, Rp, Lc, a.ce.at(idx[0], idx[1], idx[2]));
+            m += s;
+            cell_states_from<D, MHD, R, AX, FINAL>(a, idx[0], idx[1], idx[2], p_cast<R>(ld(f - 1, m - s)), p_cast<R>(ld(f, m)),
+                                                   p_cast<R>(nxt), Lc, Rc);
+            nxt = ld(f + 2, m + 2 * s);
+            face_flux<MHD, R, AX>(a, Rp, Lc, m);
             Rp = Rc;
         }
     }
