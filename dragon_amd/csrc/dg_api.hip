@@ -248,45 +248,59 @@ long dg_face_count(const dg_grid* g) { return g && g->d.mhd && g->d.dim >= 2 ? g
 const char* dg_last_error(const dg_grid* g) { return g ? g->err.c_str() : "null handle"; }
 void* dg_stream(dg_grid* g) { return g ? (void*)g->s : nullptr; }
 
+// A staging buffer that is freed on every return path.  release() frees it
+// with the error check of the success path.
+struct Staging {
+    double* p = nullptr;
+    ~Staging() {
+        if (p) (void)hipFree(p);
+    }
+    hipError_t release() {
+        double* q = p;
+        p = nullptr;
+        return q ? hipFree(q) : hipSuccess;
+    }
+};
+
 int dg_upload(dg_grid* g, const double* w, int stride, const double* B) {
     if (!g || !w || stride < g->nf) return DG_ERR_ARG;
     g->cfl_cached = false;
     const long nc = g->ce.size;
-    double* stage = nullptr;
-    HIPCHK(hipMalloc((void**)&stage, sizeof(double) * nc * stride));
-    HIPCHK(hipMemcpyAsync(stage, w, sizeof(double) * nc * stride, hipMemcpyHostToDevice, g->s));
-    launch_aos_to_soa(stage, stride, 8, soa(g->W, nc), nc, g->s);
+    Staging stage;
+    HIPCHK(hipMalloc((void**)&stage.p, sizeof(double) * nc * stride));
+    HIPCHK(hipMemcpyAsync(stage.p, w, sizeof(double) * nc * stride, hipMemcpyHostToDevice, g->s));
+    launch_aos_to_soa(stage.p, stride, 8, soa(g->W, nc), nc, g->s);
     if (B && g->B) {
         const long nfc = g->fe.size;
-        double* sb = nullptr;
-        HIPCHK(hipMalloc((void**)&sb, sizeof(double) * 3 * nfc));
-        HIPCHK(hipMemcpyAsync(sb, B, sizeof(double) * 3 * nfc, hipMemcpyHostToDevice, g->s));
-        launch_aos_to_soa(sb, 3, 3, soa(g->B, nfc), nfc, g->s);
+        Staging sb;
+        HIPCHK(hipMalloc((void**)&sb.p, sizeof(double) * 3 * nfc));
+        HIPCHK(hipMemcpyAsync(sb.p, B, sizeof(double) * 3 * nfc, hipMemcpyHostToDevice, g->s));
+        launch_aos_to_soa(sb.p, 3, 3, soa(g->B, nfc), nfc, g->s);
         HIPCHK(hipStreamSynchronize(g->s));
-        HIPCHK(hipFree(sb));
+        HIPCHK(sb.release());
     }
     HIPCHK(hipStreamSynchronize(g->s));
-    HIPCHK(hipFree(stage));
+    HIPCHK(stage.release());
     return DG_OK;
 }
 
 int dg_download(dg_grid* g, double* w, int stride, double* B) {
     if (!g || !w || stride < g->nf) return DG_ERR_ARG;
     const long nc = g->ce.size;
-    double* stage = nullptr;
-    HIPCHK(hipMalloc((void**)&stage, sizeof(double) * nc * stride));
-    launch_soa_to_aos(soa(g->W, nc), g->d.mhd ? 8 : 5, stage, stride, nc, g->s);
-    HIPCHK(hipMemcpyAsync(w, stage, sizeof(double) * nc * stride, hipMemcpyDeviceToHost, g->s));
+    Staging stage;
+    HIPCHK(hipMalloc((void**)&stage.p, sizeof(double) * nc * stride));
+    launch_soa_to_aos(soa(g->W, nc), g->d.mhd ? 8 : 5, stage.p, stride, nc, g->s);
+    HIPCHK(hipMemcpyAsync(w, stage.p, sizeof(double) * nc * stride, hipMemcpyDeviceToHost, g->s));
     HIPCHK(hipStreamSynchronize(g->s));
-    HIPCHK(hipFree(stage));
+    HIPCHK(stage.release());
     if (B && g->B) {
         const long nfc = g->fe.size;
-        double* sb = nullptr;
-        HIPCHK(hipMalloc((void**)&sb, sizeof(double) * 3 * nfc));
-        launch_soa_to_aos(soa(g->B, nfc), 3, sb, 3, nfc, g->s);
-        HIPCHK(hipMemcpyAsync(B, sb, sizeof(double) * 3 * nfc, hipMemcpyDeviceToHost, g->s));
+        Staging sb;
+        HIPCHK(hipMalloc((void**)&sb.p, sizeof(double) * 3 * nfc));
+        launch_soa_to_aos(soa(g->B, nfc), 3, sb.p, 3, nfc, g->s);
+        HIPCHK(hipMemcpyAsync(B, sb.p, sizeof(double) * 3 * nfc, hipMemcpyDeviceToHost, g->s));
         HIPCHK(hipStreamSynchronize(g->s));
-        HIPCHK(hipFree(sb));
+        HIPCHK(sb.release());
     }
     return DG_OK;
 }
