@@ -722,7 +722,7 @@ static long batch_items(const dg_batch* b, const PatchDesc& q, int kind) {
     };
     switch (kind) {
         case BK_BCC_X: return sides * (ny + 6);
-        case BK_BCC_Y: return sides * (nx + 6);
+        case BK_BCC_Y: return 2 * (nx + 6) * 3;  // the contiguous axis: one item per ghost cell, both sides
         case BK_BCF_X: return sides * (ny + 7);
         case BK_BCF_Y: return sides * (nx + 7);
         case BK_CELLS1: return (nx + 2) * (ny + 2);

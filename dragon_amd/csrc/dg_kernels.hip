@@ -1708,7 +1708,7 @@ KB __global__ void kb_bc(BatchArgs A, BlockTable bt) {
     constexpr int G = 3;
     const long cnt = v.ce.n[AX == 0 ? 1 : 0] + 2 * G + (FACES ? 1 : 0);
     const bool periodic = A.bc_kind == 1;
-    if (t >= (periodic ? 1 : 2) * cnt) return;
+    if (t >= (!FACES && AX == 1 ? 2 * G : periodic ? 1 : 2) * cnt) return;
     BCArgs b{};
     b.W = v.W;
     for (int c = 0; c < 3; c++) b.B[c] = v.B[c];
@@ -1716,6 +1716,15 @@ KB __global__ void kb_bc(BatchArgs A, BlockTable bt) {
     b.axis = AX;
     b.lo[0] = -G; b.hi[0] = (int)(-G + cnt);
     b.lo[1] = 0; b.hi[1] = 1;
+    if constexpr (!FACES && AX == 1) {
+        // The cell pass along y (the contiguous axis), one item per ghost
+        // cell, both sides: patches hold at least 4 cells along y, so every
+        // cell a ghost reads is interior and the g loop has no order to keep.
+        const int side = (int)(t / (cnt * G));
+        const long r = t % (cnt * G);
+        bc_cell(b, b.lo[0] + (int)(r / G), 0, (int)(r % G) + 1, side);
+        return;
+    }
     b.side = periodic ? 2 : (int)(t / cnt);
     if (FACES) bc_faces_body(b, t % cnt);
     else bc_cells_body(b, t % cnt);
