@@ -2,6 +2,7 @@
 
     python -m dragon_amd.profiling stats  KT_DIR OUT.md --command "..."
     python -m dragon_amd.profiling traffic FETCH_DIR WRITE_DIR OUT.json --steps 3 --forms fp64,mixed
+    python -m dragon_amd.profiling valu PMC_DIR OUT.md --command "..."
 
 ``stats`` turns a ``rocprofv3 --kernel-trace --stats --output-format csv``
 directory into a Markdown table (per kernel: calls, average and total device
@@ -13,6 +14,14 @@ step for each precision form (run bench.py with --no-batch).  The counters
 are scaled by what they read for bench.py's copy probe, torch's negation
 kernel over 1 GiB (1 GiB read, 1 GiB written): factor = known bytes /
 counted bytes.
+
+``valu`` turns one ``rocprofv3 --pmc SQ_WAVES SQ_INSTS_VALU SQ_INSTS_VMEM_RD
+SQ_INSTS_SALU SQ_WAVE_CYCLES SQ_WAIT_ANY SQ_ACTIVE_INST_VALU SQ_BUSY_CYCLES``
+run into a table per kernel class and form: waves, instructions per wave,
+device time per launch (from the counter run's own timestamps) and the VALU
+issue share, SQ_INSTS_VALU x 4 cycles (a wave64 instruction on a 16-lane
+SIMD) over time x 2.4 GHz x 1024 SIMDs -- a lower bound when the engine
+clock runs below its 2.4 GHz peak.
 """
 from __future__ import annotations
 
@@ -147,6 +156,45 @@ def traffic(fetch_dir, write_dir, steps, forms):
     return out
 
 
+VALU_CYCLES, CLOCK_HZ, SIMDS = 4, 2.4e9, 1024
+
+
+def valu_table(pmc_dir: str, command: str) -> str:
+    disp = {}
+    for f in _find(pmc_dir, "counter_collection.csv"):
+        for r in csv.DictReader(open(f)):
+            did = int(_col(r, "Dispatch_Id"))
+            d = disp.setdefault(did, {"name": _col(r, "Kernel_Name"), "ns": int(_col(r, "End_Timestamp")) - int(_col(r, "Start_Timestamp"))})
+            c = _col(r, "Counter_Name")
+            d[c] = d.get(c, 0.0) + float(_col(r, "Counter_Value"))
+    agg = collections.OrderedDict()
+    for did in sorted(disp):
+        d = disp[did]
+        cls = kernel_class(d["name"])
+        if cls in STEP_CLASSES_EXCLUDED or cls == "other" or cls.startswith("batch"):
+            continue
+        form = "fp64" if "p64::" in d["name"] else "mixed" if "p32::" in d["name"] else "?"
+        a = agg.setdefault((form, cls), collections.defaultdict(float))
+        a["launches"] += 1
+        a["ns"] += d["ns"]
+        for k, v in d.items():
+            if k.startswith("SQ_"):
+                a[k] += v
+    lines = [f"Command: `{command}`", "",
+             "| form | kernel class | launches | waves/launch | VALU/wave | VMEM rd/wave | SALU/wave | ms/launch | VALU issue share | wave cycles waiting |",
+             "|---|---|---|---|---|---|---|---|---|---|"]
+    for (form, cls), a in agg.items():
+        n, w = a["launches"], a.get("SQ_WAVES", 0.0)
+        if not w:
+            continue
+        share = a.get("SQ_INSTS_VALU", 0.0) * VALU_CYCLES / (a["ns"] * 1e-9 * CLOCK_HZ * SIMDS)
+        wait = a.get("SQ_WAIT_ANY", 0.0) / a["SQ_WAVE_CYCLES"] if a.get("SQ_WAVE_CYCLES") else float("nan")
+        lines.append(f"| {form} | {cls} | {int(n)} | {w / n:.0f} | {a.get('SQ_INSTS_VALU', 0.0) / w:.0f} | "
+                     f"{a.get('SQ_INSTS_VMEM_RD', 0.0) / w:.0f} | {a.get('SQ_INSTS_SALU', 0.0) / w:.0f} | "
+                     f"{a['ns'] / n / 1e6:.2f} | {share:.2f} | {wait:.2f} |")
+    return "\n".join(lines) + "\n"
+
+
 def main(argv=None):
     ap = argparse.ArgumentParser(prog="python -m dragon_amd.profiling")
     sub = ap.add_subparsers(dest="cmd", required=True)
@@ -160,8 +208,15 @@ def main(argv=None):
     a2.add_argument("out")
     a2.add_argument("--steps", type=int, required=True, help="steps per form in the counted run (warmup + timed)")
     a2.add_argument("--forms", default="fp64,mixed")
+    a3 = sub.add_parser("valu")
+    a3.add_argument("pmc_dir")
+    a3.add_argument("out")
+    a3.add_argument("--command", default="")
     a = ap.parse_args(argv)
-    if a.cmd == "stats":
+    if a.cmd == "valu":
+        with open(a.out, "w") as f:
+            f.write(valu_table(a.pmc_dir, a.command))
+    elif a.cmd == "stats":
         text = stats_table(a.kt_dir, a.command)
         with open(a.out, "w") as f:
             f.write(text)

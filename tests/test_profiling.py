@@ -47,3 +47,21 @@ def test_stats_table(tmp_path):
               [("void dg::k_apply<3, true>(...)", 4, 4e6, 1e6, 60.0, 9e5, 1.1e6, 1.0)])
     t = P.stats_table(str(tmp_path / "kt"), "rocprofv3 ... -- python3 bench.py")
     assert "| apply | 4 | 1000.0 | 4.00 | 60.00 |" in t
+
+
+def test_valu_table(tmp_path):
+    hdr = ["Dispatch_Id", "Kernel_Name", "Counter_Name", "Counter_Value", "Start_Timestamp", "End_Timestamp"]
+    k = "void dg::p32::k_sweep_wave<3, true, float, 2, false>(dg::p32::SweepArgs)"
+    rows = []
+    for did in (1, 2):  # two launches of 1 ms each, 256000 waves of 1200 VALU instructions
+        for c, v in (("SQ_WAVES", 256000), ("SQ_INSTS_VALU", 256000 * 1200), ("SQ_WAVE_CYCLES", 1000), ("SQ_WAIT_ANY", 250)):
+            rows.append((did, k, c, v, 0, 1_000_000))
+    rows.append((3, "void dg::p64::k_init_B<3>(...)", "SQ_WAVES", 5, 0, 10))
+    write_csv(str(tmp_path / "p" / "x" / "p_counter_collection.csv"), hdr, rows)
+    text = P.valu_table(str(tmp_path / "p"), "cmd")
+    line = [l for l in text.splitlines() if "pred_sweep_z" in l]
+    assert len(line) == 1 and "init" not in text
+    cells = [c.strip() for c in line[0].strip("|").split("|")]
+    assert cells[0] == "mixed" and cells[2] == "2" and cells[3] == "256000" and cells[4] == "1200"
+    # 256000 * 1200 * 4 cycles / (1 ms * 2.4 GHz * 1024 SIMDs) = 0.5
+    assert cells[8] == "0.50" and cells[9] == "0.25"
