@@ -54,6 +54,23 @@ def test_c2_kelvin_helmholtz_1024_500_steps():
     assert all(v <= CONS for v in d.values()), d
 
 
+def test_c2_mixed_vs_fp64_1024_500_steps():
+    """C2 in full, both forms: the mixed form's relative L1 per variable
+    against the FP64 form (bit-identical to the oracle) after 500 steps."""
+    pb = P.kelvin_helmholtz_2d(1024)
+    out = []
+    for prec in (0, 1):
+        G = gpu_grid(pb, prec)
+        steps, fails = G.advance(1e9, True, 500)
+        assert steps == 500
+        out.append(interior(G.get_w(), G.g, pb[0]))
+        del G
+    d = {nm: float(np.abs(out[1][..., q] - out[0][..., q]).sum() / np.abs(out[0][..., q]).sum())
+         for q, nm in enumerate(["rho", "p", "vx", "vy", "vz"]) if np.abs(out[0][..., q]).sum() > 0}
+    print(f"C2 1024^2 mixed vs FP64 relative L1 per variable after 500 steps: { {k: f'{v:.3g}' for k, v in d.items()} }", flush=True)
+    assert d["rho"] <= 1e-5
+
+
 def rel_l1_vars(a, b):
     names = ["rho", "p", "vx", "vy", "vz", "Bx", "By", "Bz"]
     out = {}

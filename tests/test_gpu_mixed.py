@@ -14,13 +14,15 @@ from helpers import compare_states, gpu_grid, oracle_grid
 pytestmark = pytest.mark.gpu
 
 
-def run_mixed(problem, steps):
+def run_mixed(problem, steps=None, t=None, nf=8):
     og = oracle_grid(problem)
     gg = gpu_grid(problem, precision=1)
-    n_o = og.advance(1e9, True, steps)
-    n_g = gg.advance(1e9, True, steps)
-    res = compare_states(gg, og, problem[0], 8)
-    divb = gg.divb()
+    if t is not None:
+        n_o, n_g = og.advance(t), gg.advance(t)
+    else:
+        n_o, n_g = og.advance(1e9, True, steps), gg.advance(1e9, True, steps)
+    res = compare_states(gg, og, problem[0], nf)
+    divb = gg.divb() if nf == 8 else 0.0
     print(problem[0], n_o, n_g, "rel L1", res["rel_l1"], "divB", divb)
     return res, divb
 
@@ -35,3 +37,22 @@ def test_mixed_turbulence_3d():
     res, divb = run_mixed(P.turbulence_3d(16), 12)
     assert res["rel_l1"][0] <= 1e-5
     assert divb <= 1e-12
+
+
+def test_mixed_sod_1d_full():
+    """C1 in full: 1D Sod, 1024 cells, HLLC + minmod + verify fallback, to t = 0.2."""
+    pb = P.sod_1d(1024)
+    res, _ = run_mixed(pb, t=pb[6]["t_end"], nf=5)
+    assert res["rel_l1"][0] <= 1e-5
+
+
+def test_mixed_kelvin_helmholtz_2d():
+    """C2 switch set (hydro CTU 4-solve, HLLC, van Leer, periodic) at 64^2, 60 steps."""
+    res, _ = run_mixed(P.kelvin_helmholtz_2d(64), 60, nf=5)
+    assert res["rel_l1"][0] <= 1e-5
+
+
+def test_mixed_hydro_3d_ctu12():
+    """3D hydro CTU 12-solve (diagonal advection) at 16^3, 10 steps."""
+    res, _ = run_mixed(P.diagonal_3d(16), 10, nf=5)
+    assert res["rel_l1"][0] <= 1e-5
