@@ -637,10 +637,10 @@ __device__ __forceinline__ void face_flux(const SweepArgs& a, P<R> l, P<R> r, lo
 }
 
 // Windowed walkers (stencil cells carried in registers) for the FP32 form;
-// the FP64 walkers are register-bound (the window takes them from 236 to 256
-// VGPRs, one wave per SIMD) and re-read the stencil.  The windowed FP64
-// instantiation exists as a check of the window's cell bookkeeping: its
-// results must equal the oracle's bit for bit.
+// the FP64 walkers are register-bound (the window takes them to 256 VGPRs,
+// one wave per SIMD): they read the cell ahead early and re-read the two
+// behind it.  The windowed FP64 instantiation exists as a check of the
+// window's cell bookkeeping: its results must equal the oracle's bit for bit.
 template <class R> constexpr bool march_window() { return sizeof(R) == 4; }
 
 // Strided axis: one thread walks `chunk` consecutive faces.
