@@ -36,8 +36,8 @@ using ET = double;
 using FSoa = SoaT<FT>;
 __host__ __device__ inline FSoa fs(const Soa& a) { return FSoa{reinterpret_cast<FT*>(a.p), a.stride}; }
 __host__ __device__ inline ET* et(double* p) { return reinterpret_cast<ET*>(p); }
-// EMF element widened to FP64 before any arithmetic
-__device__ __forceinline__ double ed(const ET* p, long i) { return double(p[i]); }
+// A stored element widened to FP64 before any arithmetic
+template <class T> __device__ __forceinline__ double ed(const T* p, long i) { return double(p[i]); }
 void ctu_hydro3(const StepCtx& c, hipStream_t s);
 
 #define KB __launch_bounds__(256)
@@ -436,7 +436,7 @@ KB __global__ void k_flux(Soa HL, Soa HR, FSoa F, Ext ce, Box box, double dt_d, 
 struct SweepArgs {
     Soa W;
     const double* B[3];
-    const double* Bh[3];
+    const ET* Bh[3];       // half-step face B (ET storage)
     const ET* E0[3];
     FSoa Fp[3];            // predictor fluxes (corrector pass input)
     FSoa Fo;               // output fluxes of this axis
@@ -543,11 +543,11 @@ __device__ __forceinline__ void ctu_correct(const SweepArgs& a, long m, long q, 
             corr.By = 0.25 * dt_dx * (ed(E0z, q + fx) - ed(E0z, q) + ed(E0z, q + fx + fy) - ed(E0z, q + fy)) + 0.5 * Wc.vy * lim_y;
         }
         U<double> uL = uadd<true>(cons_c<true>(L, ph), corr), uR = uadd<true>(cons_c<true>(Rr, ph), corr);
-        const double* Bh = a.Bh[AX];
+        const ET* Bh = a.Bh[AX];
         const long e = fe.s[AX];
-        if (AX == 0) { uL.Bx = Bh[q]; uR.Bx = Bh[q + e]; }
-        else if (AX == 1) { uL.By = Bh[q]; uR.By = Bh[q + e]; }
-        else { uL.Bz = Bh[q]; uR.Bz = Bh[q + e]; }
+        if (AX == 0) { uL.Bx = ed(Bh, q); uR.Bx = ed(Bh, q + e); }
+        else if (AX == 1) { uL.By = ed(Bh, q); uR.By = ed(Bh, q + e); }
+        else { uL.Bz = ed(Bh, q); uR.Bz = ed(Bh, q + e); }
         L = prim_c<true>(uL, ph);
         Rr = prim_c<true>(uR, ph);
     }
@@ -1000,9 +1000,9 @@ static void emf3_tiled(const FSoa& FX, const FSoa& FY, const FSoa& FZ, const Soa
 
 // Faraday (B_fields.cpp:21-66) fused with the clone: Bout = Bin everywhere,
 // minus dt*curl(E) on the faces of the reference's ranges (ghost width gg).
-template <int D>
+template <int D, class BO>
 __device__ __forceinline__ void faraday_body(const ET* Ex, const ET* Ey, const ET* Ez, const double* Bix, const double* Biy,
-                                             const double* Biz, double* Box_, double* Boy, double* Boz, const Ext& fe, int gg, double dt_dx,
+                                             const double* Biz, BO* Box_, BO* Boy, BO* Boz, const Ext& fe, int gg, double dt_dx,
                                              double dt_dy, double dt_dz, long m) {
     if (m >= fe.size) return;
     double bx = Bix[m], by = Biy[m], bz = Biz[m];
@@ -1026,13 +1026,15 @@ __device__ __forceinline__ void faraday_body(const ET* Ex, const ET* Ey, const E
         if (ixl && jy && kzl) by -= (ed(Ex, m + fe.s[2]) - ed(Ex, m)) * dt_dz - (ed(Ez, m + fe.s[0]) - ed(Ez, m)) * dt_dx;
         if (ixl && jyl && kz) bz -= (ed(Ey, m + fe.s[0]) - ed(Ey, m)) * dt_dx - (ed(Ex, m + fe.s[1]) - ed(Ex, m)) * dt_dy;
     }
-    Box_[m] = bx; Boy[m] = by; Boz[m] = bz;
+    Box_[m] = BO(bx); Boy[m] = BO(by); Boz[m] = BO(bz);
 }
-template <int D>
+// The final update writes FP64 face B; the half-step one (gg = 1) writes the
+// half-step faces in ET.
+template <int D, class BO>
 KB __global__ void k_faraday(const ET* Ex, const ET* Ey, const ET* Ez, const double* Bix, const double* Biy,
-                             const double* Biz, double* Box_, double* Boy, double* Boz, Ext fe, int gg, double dt_dx, double dt_dy,
+                             const double* Biz, BO* Box_, BO* Boy, BO* Boz, Ext fe, int gg, double dt_dx, double dt_dy,
                              double dt_dz) {
-    faraday_body<D>(Ex, Ey, Ez, Bix, Biy, Biz, Box_, Boy, Boz, fe, gg, dt_dx, dt_dy, dt_dz, tid());
+    faraday_body<D, BO>(Ex, Ey, Ez, Bix, Biy, Biz, Box_, Boy, Boz, fe, gg, dt_dx, dt_dy, dt_dz, tid());
 }
 
 // computeBodyField + shouldProtectThermal (B_fields.cpp:70-114, production
@@ -1053,20 +1055,21 @@ __device__ __forceinline__ P<double> body_field(P<double> w, double cbx, double 
     w.Bx = cbx; w.By = cby; w.Bz = cbz;
     return w;
 }
-__device__ __forceinline__ void cell_B(const double* Bx, const double* By, const double* Bz, const Ext& fe, int D, int i, int j, int k,
+template <class T>
+__device__ __forceinline__ void cell_B(const T* Bx, const T* By, const T* Bz, const Ext& fe, int D, int i, int j, int k,
                                        double& x, double& y, double& z) {
     const long m = fe.at(i, j, k);
-    x = (Bx[m] + Bx[m + fe.s[0]]) * 0.5;
-    y = (By[m] + By[m + fe.s[1]]) * 0.5;
-    z = D == 3 ? (Bz[m] + Bz[m + fe.s[2]]) * 0.5 : Bz[m];
+    x = (ed(Bx, m) + ed(Bx, m + fe.s[0])) * 0.5;
+    y = (ed(By, m) + ed(By, m + fe.s[1])) * 0.5;
+    z = D == 3 ? (ed(Bz, m) + ed(Bz, m + fe.s[2])) * 0.5 : ed(Bz, m);
 }
 
 // Half-step state for the CTU corrector (CTU.cpp:57-61, 154-158): applyFluxes
 // at dt/2 over cells [-1, n+1), body fields from the half-step faces, then
 // Ehalf = cross(B, v) over cells [-1, n].
 template <int D>
-__device__ __forceinline__ void whalf_body(const Soa& W, const FSoa& FX, const FSoa& FY, const FSoa& FZ, const double* Bhx, const double* Bhy,
-                                           const double* Bhz, const Ext& ce, const Ext& fe, const Box& box, double hx, double hy, double hz,
+__device__ __forceinline__ void whalf_body(const Soa& W, const FSoa& FX, const FSoa& FY, const FSoa& FZ, const ET* Bhx, const ET* Bhy,
+                                           const ET* Bhz, const Ext& ce, const Ext& fe, const Box& box, double hx, double hy, double hz,
                                            const Phys& ph, ET* Ehx, ET* Ehy, ET* Ehz, unsigned long long* err, long t) {
     int i, j, k;
     if (!box.point(t, i, j, k)) return;
@@ -1087,7 +1090,7 @@ __device__ __forceinline__ void whalf_body(const Soa& W, const FSoa& FX, const F
     Ehx[e] = ex; Ehy[e] = ey; Ehz[e] = ez;
 }
 template <int D>
-KB __global__ void k_whalf(Soa W, FSoa FX, FSoa FY, FSoa FZ, const double* Bhx, const double* Bhy, const double* Bhz, Ext ce,
+KB __global__ void k_whalf(Soa W, FSoa FX, FSoa FY, FSoa FZ, const ET* Bhx, const ET* Bhy, const ET* Bhz, Ext ce,
                            Ext fe, Box box, double hx, double hy, double hz, Phys ph, ET* Ehx, ET* Ehy, ET* Ehz,
                            unsigned long long* err) {
     whalf_body<D>(W, FX, FY, FZ, Bhx, Bhy, Bhz, ce, fe, box, hx, hy, hz, ph, Ehx, Ehy, Ehz, err, tid());
@@ -1095,8 +1098,8 @@ KB __global__ void k_whalf(Soa W, FSoa FX, FSoa FY, FSoa FZ, const double* Bhx, 
 
 // ctu_sweep_MHD corrections (CTU.cpp:65-129 3D, 162-202 2D), in place.
 template <int D>
-KB __global__ void k_ctu_mhd(Soa W, const double* Bx, const double* By, const double* Bz, const double* Bhx, const double* Bhy,
-                             const double* Bhz, const ET* E0x, const ET* E0y, const ET* E0z, FSoa FX, FSoa FY, FSoa FZ,
+KB __global__ void k_ctu_mhd(Soa W, const double* Bx, const double* By, const double* Bz, const ET* Bhx, const ET* Bhy,
+                             const ET* Bhz, const ET* E0x, const ET* E0y, const ET* E0z, FSoa FX, FSoa FY, FSoa FZ,
                              Soa XL, Soa XR, Soa YL, Soa YR, Soa ZL, Soa ZR, Ext ce, Ext fe, Box box, double dt_dx, double dt_dy,
                              double dt_dz, Phys ph) {
     int i, j, k;
@@ -1110,9 +1113,9 @@ KB __global__ void k_ctu_mhd(Soa W, const double* Bx, const double* By, const do
     auto apply = [&](Soa& HL, Soa& HR, U<double> corr, int ax) {
         U<double> uL = uadd<true>(cons<true>(load_p<double>(HL, m, true), ph), corr);
         U<double> uR = uadd<true>(cons<true>(load_p<double>(HR, m, true), ph), corr);
-        if (ax == 0) { uL.Bx = Bhx[q]; uR.Bx = Bhx[q + fx]; }
-        else if (ax == 1) { uL.By = Bhy[q]; uR.By = Bhy[q + fy]; }
-        else { uL.Bz = Bhz[q]; uR.Bz = Bhz[q + fz]; }
+        if (ax == 0) { uL.Bx = ed(Bhx, q); uR.Bx = ed(Bhx, q + fx); }
+        else if (ax == 1) { uL.By = ed(Bhy, q); uR.By = ed(Bhy, q + fy); }
+        else { uL.Bz = ed(Bhz, q); uR.Bz = ed(Bhz, q + fz); }
         store_p<double>(HL, m, prim<true>(uL, ph), true);
         store_p<double>(HR, m, prim<true>(uR, ph), true);
     };
@@ -1379,7 +1382,7 @@ static void sweep_t(const StepCtx& c, const Box& b, const FSoa& out, int stage, 
     SweepArgs a{};
     a.W = c.W;
     for (int q = 0; q < 3; q++) {
-        a.B[q] = c.B[q]; a.Bh[q] = c.Bh[q]; a.E0[q] = et(c.E0[q]); a.Fp[q] = fs(c.Fp[q]);
+        a.B[q] = c.B[q]; a.Bh[q] = et(c.Bh[q]); a.E0[q] = et(c.E0[q]); a.Fp[q] = fs(c.Fp[q]);
         a.lo[q] = b.lo[q]; a.hi[q] = b.hi[q];
         a.dt_d[q] = c.dt / c.d[q];
     }
@@ -1444,12 +1447,12 @@ static void step_fused(const StepCtx& c, hipStream_t s) {
             else
                 emf3_tiled<true>(fs(c.Fp[0]), fs(c.Fp[1]), fs(c.Fp[2]), c.W, none, ce, c.fe, eb, 1, et(c.E0[0]), et(c.E0[1]), et(c.E0[2]), s);
             mark(c, K_EMF_PRED);
-            hipLaunchKernelGGL((k_faraday<D>), dim3(blocks(c.fe.size)), dim3(256), 0, s, et(c.E0[0]), et(c.E0[1]), et(c.E0[2]), c.B[0], c.B[1],
-                               c.B[2], c.Bh[0], c.Bh[1], c.Bh[2], c.fe, 1, 0.5 * dt_d[0], 0.5 * dt_d[1], 0.5 * dt_d[2]);
+            hipLaunchKernelGGL((k_faraday<D, ET>), dim3(blocks(c.fe.size)), dim3(256), 0, s, et(c.E0[0]), et(c.E0[1]), et(c.E0[2]), c.B[0], c.B[1],
+                               c.B[2], et(c.Bh[0]), et(c.Bh[1]), et(c.Bh[2]), c.fe, 1, 0.5 * dt_d[0], 0.5 * dt_d[1], 0.5 * dt_d[2]);
             mark(c, K_FARADAY_HALF);
             Box cb = make_box(D, -1, ce.n[0] + 1, -1, ce.n[1] + 1, -1, ce.n[2] + 1);
-            hipLaunchKernelGGL((k_whalf<D>), dim3(blocks(cb.count())), dim3(256), 0, s, c.W, fs(c.Fp[0]), fs(c.Fp[1]), fs(c.Fp[2]), c.Bh[0],
-                               c.Bh[1], c.Bh[2], ce, c.fe, cb, 0.5 * dt_d[0], 0.5 * dt_d[1], 0.5 * dt_d[2], c.ph, et(c.Eh[0]), et(c.Eh[1]),
+            hipLaunchKernelGGL((k_whalf<D>), dim3(blocks(cb.count())), dim3(256), 0, s, c.W, fs(c.Fp[0]), fs(c.Fp[1]), fs(c.Fp[2]), et(c.Bh[0]),
+                               et(c.Bh[1]), et(c.Bh[2]), ce, c.fe, cb, 0.5 * dt_d[0], 0.5 * dt_d[1], 0.5 * dt_d[2], c.ph, et(c.Eh[0]), et(c.Eh[1]),
                                et(c.Eh[2]), c.err);
             mark(c, K_WHALF);
         }
@@ -1472,7 +1475,7 @@ static void step_fused(const StepCtx& c, hipStream_t s) {
         else
             emf3_tiled<false>(fs(c.F[0]), fs(c.F[1]), fs(c.F[2]), c.W, eh, ce, c.fe, eb, c.ctu, et(c.E0[0]), et(c.E0[1]), et(c.E0[2]), s);
         mark(c, K_EMF);
-        hipLaunchKernelGGL((k_faraday<D>), dim3(blocks(c.fe.size)), dim3(256), 0, s, et(c.E0[0]), et(c.E0[1]), et(c.E0[2]), c.B[0], c.B[1], c.B[2],
+        hipLaunchKernelGGL((k_faraday<D, double>), dim3(blocks(c.fe.size)), dim3(256), 0, s, et(c.E0[0]), et(c.E0[1]), et(c.E0[2]), c.B[0], c.B[1], c.B[2],
                            c.Bn[0], c.Bn[1], c.Bn[2], c.fe, 0, dt_d[0], dt_d[1], dt_d[2]);
         mark(c, K_FARADAY);
     }
@@ -1525,16 +1528,16 @@ static void step_t(const StepCtx& c, hipStream_t s) {
                 hipLaunchKernelGGL((k_emf3<true>), dim3(blocks(eb.count())), dim3(256), 0, s, fs(c.F[0]), fs(c.F[1]), fs(c.F[2]), c.W, none, ce, c.fe,
                                    eb, 1, et(c.E0[0]), et(c.E0[1]), et(c.E0[2]));
             mark(c, K_EMF_PRED);
-            hipLaunchKernelGGL((k_faraday<D>), dim3(blocks(c.fe.size)), dim3(256), 0, s, et(c.E0[0]), et(c.E0[1]), et(c.E0[2]), c.B[0], c.B[1],
-                               c.B[2], c.Bh[0], c.Bh[1], c.Bh[2], c.fe, 1, 0.5 * dt_d[0], 0.5 * dt_d[1], 0.5 * dt_d[2]);
+            hipLaunchKernelGGL((k_faraday<D, ET>), dim3(blocks(c.fe.size)), dim3(256), 0, s, et(c.E0[0]), et(c.E0[1]), et(c.E0[2]), c.B[0], c.B[1],
+                               c.B[2], et(c.Bh[0]), et(c.Bh[1]), et(c.Bh[2]), c.fe, 1, 0.5 * dt_d[0], 0.5 * dt_d[1], 0.5 * dt_d[2]);
             mark(c, K_FARADAY_HALF);
             Box cb = make_box(D, -1, ce.n[0] + 1, -1, ce.n[1] + 1, -1, ce.n[2] + 1);
-            hipLaunchKernelGGL((k_whalf<D>), dim3(blocks(cb.count())), dim3(256), 0, s, c.W, fs(c.F[0]), fs(c.F[1]), fs(c.F[2]), c.Bh[0], c.Bh[1],
-                               c.Bh[2], ce, c.fe, cb, 0.5 * dt_d[0], 0.5 * dt_d[1], 0.5 * dt_d[2], c.ph, et(c.Eh[0]), et(c.Eh[1]), et(c.Eh[2]),
+            hipLaunchKernelGGL((k_whalf<D>), dim3(blocks(cb.count())), dim3(256), 0, s, c.W, fs(c.F[0]), fs(c.F[1]), fs(c.F[2]), et(c.Bh[0]), et(c.Bh[1]),
+                               et(c.Bh[2]), ce, c.fe, cb, 0.5 * dt_d[0], 0.5 * dt_d[1], 0.5 * dt_d[2], c.ph, et(c.Eh[0]), et(c.Eh[1]), et(c.Eh[2]),
                                c.err);
             mark(c, K_WHALF);
-            hipLaunchKernelGGL((k_ctu_mhd<D>), dim3(blocks(cb.count())), dim3(256), 0, s, c.W, c.B[0], c.B[1], c.B[2], c.Bh[0], c.Bh[1],
-                               c.Bh[2], et(c.E0[0]), et(c.E0[1]), et(c.E0[2]), fs(c.F[0]), fs(c.F[1]), fs(c.F[2]), c.HS[0], c.HS[1], c.HS[2], c.HS[3],
+            hipLaunchKernelGGL((k_ctu_mhd<D>), dim3(blocks(cb.count())), dim3(256), 0, s, c.W, c.B[0], c.B[1], c.B[2], et(c.Bh[0]), et(c.Bh[1]),
+                               et(c.Bh[2]), et(c.E0[0]), et(c.E0[1]), et(c.E0[2]), fs(c.F[0]), fs(c.F[1]), fs(c.F[2]), c.HS[0], c.HS[1], c.HS[2], c.HS[3],
                                c.HS[4], c.HS[5], ce, c.fe, cb, dt_d[0], dt_d[1], dt_d[2], c.ph);
             mark(c, K_CTU);
         } else if constexpr (D == 2) {
@@ -1580,7 +1583,7 @@ static void step_t(const StepCtx& c, hipStream_t s) {
             hipLaunchKernelGGL((k_emf3<false>), dim3(blocks(eb.count())), dim3(256), 0, s, fs(c.F[0]), fs(c.F[1]), fs(c.F[2]), c.W, eh, ce, c.fe, eb,
                                c.ctu, et(c.E0[0]), et(c.E0[1]), et(c.E0[2]));
         mark(c, K_EMF);
-        hipLaunchKernelGGL((k_faraday<D>), dim3(blocks(c.fe.size)), dim3(256), 0, s, et(c.E0[0]), et(c.E0[1]), et(c.E0[2]), c.B[0], c.B[1], c.B[2],
+        hipLaunchKernelGGL((k_faraday<D, double>), dim3(blocks(c.fe.size)), dim3(256), 0, s, et(c.E0[0]), et(c.E0[1]), et(c.E0[2]), c.B[0], c.B[1], c.B[2],
                            c.Bn[0], c.Bn[1], c.Bn[2], c.fe, 0, dt_d[0], dt_d[1], dt_d[2]);
         mark(c, K_FARADAY);
     }
@@ -1649,7 +1652,8 @@ void ctu_hydro3(const StepCtx& c, hipStream_t s) {
 struct PatchView {
     Ext ce, fe;
     Soa W, Wn;
-    double *B[3], *Bn[3], *Bh[3];
+    double *B[3], *Bn[3];
+    ET* Bh[3];  // half-step faces
     ET *E0[3], *Eh[3];
     FSoa Fp[3], F[3];
     double dx, dy;
@@ -1667,7 +1671,7 @@ __device__ __forceinline__ PatchView patch_view(const BatchArgs& A, int p) {
     for (int c = 0; c < 3; c++) {
         v.B[c] = A.B[par] + 3 * P.of + c * P.nfc;
         v.Bn[c] = A.B[1 - par] + 3 * P.of + c * P.nfc;
-        v.Bh[c] = A.Bh + 3 * P.of + c * P.nfc;
+        v.Bh[c] = et(A.Bh) + 3 * P.of + c * P.nfc;
         v.E0[c] = et(A.E0) + 3 * P.of + c * P.nfc;  // EMF pools hold ET elements at the face offsets
         v.Eh[c] = et(A.Eh) + 3 * P.of + c * P.nfc;
     }
@@ -1829,8 +1833,8 @@ KB __global__ void kb_faraday(BatchArgs A, BlockTable bt) {
     const PatchView v = patch_view(A, p);
     const double dt = A.dt[p], f = HALF ? 0.5 : 1.0;
     const double dx = f * (dt / v.dx), dy = f * (dt / v.dy), dz = f * (dt / 1.0);
-    if (HALF) faraday_body<2>(v.E0[0], v.E0[1], v.E0[2], v.B[0], v.B[1], v.B[2], v.Bh[0], v.Bh[1], v.Bh[2], v.fe, 1, dx, dy, dz, t);
-    else faraday_body<2>(v.E0[0], v.E0[1], v.E0[2], v.B[0], v.B[1], v.B[2], v.Bn[0], v.Bn[1], v.Bn[2], v.fe, 0, dx, dy, dz, t);
+    if (HALF) faraday_body<2, ET>(v.E0[0], v.E0[1], v.E0[2], v.B[0], v.B[1], v.B[2], v.Bh[0], v.Bh[1], v.Bh[2], v.fe, 1, dx, dy, dz, t);
+    else faraday_body<2, double>(v.E0[0], v.E0[1], v.E0[2], v.B[0], v.B[1], v.B[2], v.Bn[0], v.Bn[1], v.Bn[2], v.fe, 0, dx, dy, dz, t);
 }
 KB __global__ void kb_whalf(BatchArgs A, BlockTable bt) {
     BATCH_ITEM(bt)
