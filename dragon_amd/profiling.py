@@ -130,12 +130,12 @@ def _split_forms(recs, forms):
     return out
 
 
-def traffic(fetch_dir, write_dir, steps, forms):
+def traffic(fetch_dir, write_dir, steps, forms, commands=()):
     fr = _pmc_records(fetch_dir, "FETCH_SIZE")
     wr = _pmc_records(write_dir, "WRITE_SIZE")
     ff, fkib = _copy_factor(fr)
     wf, wkib = _copy_factor(wr)
-    out = {"unit": "bytes", "steps_per_form": steps,
+    out = {"unit": "bytes", "steps_per_form": steps, "commands": list(commands),
            "copy_probe": {"bytes_each_way": 1 << 30, "fetch_kib": fkib, "write_kib": wkib,
                           "fetch_factor": ff, "write_factor": wf}}
     fsplit, wsplit = _split_forms(fr, forms), _split_forms(wr, forms)
@@ -208,6 +208,7 @@ def main(argv=None):
     a2.add_argument("out")
     a2.add_argument("--steps", type=int, required=True, help="steps per form in the counted run (warmup + timed)")
     a2.add_argument("--forms", default="fp64,mixed")
+    a2.add_argument("--command", action="append", default=[], help="the counter runs' commands, recorded in the output")
     a3 = sub.add_parser("valu")
     a3.add_argument("pmc_dir")
     a3.add_argument("out")
@@ -221,7 +222,7 @@ def main(argv=None):
         with open(a.out, "w") as f:
             f.write(text)
     else:
-        res = traffic(a.fetch_dir, a.write_dir, a.steps, a.forms.split(","))
+        res = traffic(a.fetch_dir, a.write_dir, a.steps, a.forms.split(","), a.command)
         with open(a.out, "w") as f:
             json.dump(res, f, indent=1)
 
