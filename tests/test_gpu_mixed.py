@@ -56,3 +56,12 @@ def test_mixed_hydro_3d_ctu12():
     """3D hydro CTU 12-solve (diagonal advection) at 16^3, 10 steps."""
     res, _ = run_mixed(P.diagonal_3d(16), 10, nf=5)
     assert res["rel_l1"][0] <= 1e-5
+
+
+@pytest.mark.parametrize("shape", [(2, 3), (2, 1, 3)])
+def test_mixed_tiny_grids(shape):
+    """The mixed form's walkers and wavefronts on grids of a few cells."""
+    from test_gpu_parity import tiny_mhd
+    res, divb = run_mixed(tiny_mhd(shape, 1), 2)
+    assert res["rel_l1"][0] <= 1e-5
+    assert divb <= 1e-12

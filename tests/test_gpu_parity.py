@@ -92,6 +92,30 @@ def test_mhd_3d_rectangular():
     run_pair(rect_mhd_3d(), steps=5)
 
 
+def tiny_mhd(shape, bc):
+    """Random MHD state (CTU + HLLD + CT, van Leer) on a grid of a few cells,
+    outflow (0) or periodic (1) on every face."""
+    g = P.ghosts_for(True)
+    w, B = P.empty_state(shape, g, True)
+    rng = np.random.default_rng(3)
+    wi = P.interior(w, g, shape)
+    wi[..., 0] = 1.0 + 0.3 * rng.random(shape)
+    wi[..., 1] = 1.0 + 0.3 * rng.random(shape)
+    wi[..., 2:5] = 0.3 * (rng.random(shape + (3,)) - 0.5)
+    P.interior(B, g, shape, extra=1)[..., 0] = 0.2
+    cfg = dict(mhd=1, gamma=5.0 / 3.0, solver=5, limiter=2, ctu=1, ct_energy=2)
+    allf = {2: 15, 3: 63}[len(shape)]
+    return (shape, tuple(1.0 / n for n in shape), cfg, [(bc, allf)], w, B, {"init_B": True})
+
+
+@pytest.mark.parametrize("shape", [(1, 1), (2, 3), (1, 1, 1), (2, 1, 3)])
+@pytest.mark.parametrize("bc", [0, 1])
+def test_tiny_grids(shape, bc):
+    """Grids of one to three cells per axis (fewer cells than ghost layers):
+    every stencil, walker, wavefront and ghost pass at its smallest."""
+    run_pair(tiny_mhd(shape, bc), steps=2)
+
+
 @pytest.mark.parametrize("nz", [2, 3])
 def test_thin_contiguous_axis(nz):
     """Ghost passes on the contiguous axis run one thread per ghost cell when
