@@ -676,17 +676,19 @@ template <class T> __device__ U<T> exact(const P<T>& L, const P<T>& R, const Phy
 }
 
 // ---- dispatch + verify-and-fallback (Flux.cpp:17-97).  Sets *fail to an
-// RF_* code when no solver yields a physical update.
+// RF_* code when no solver yields a physical update.  The runtime choice of
+// Flux.cpp:34-53: MHD takes HLL, HLLE, else HLLD; hydro takes HLLC for HLLX
+// and HLLC, HLL, HLLE, else the exact solver (HLLD included: it is MHD only).
 template <bool MHD, class T> __device__ __forceinline__ U<T> solve(const P<T>& L, const P<T>& R, const Phys& ph) {
     if constexpr (MHD) {
         if (ph.solver == RS_HLL) return hll<true>(L, R, ph);
         if (ph.solver == RS_HLLE) return hlle<true>(L, R, ph);
         return hlld(L, R, ph);
     } else {
+        if (ph.solver == RS_HLLX || ph.solver == RS_HLLC) return hllc(L, R, ph);
         if (ph.solver == RS_HLL) return hll<false>(L, R, ph);
         if (ph.solver == RS_HLLE) return hlle<false>(L, R, ph);
-        if (ph.solver == RS_EXACT) return exact(L, R, ph);
-        return hllc(L, R, ph);
+        return exact(L, R, ph);
     }
 }
 template <bool MHD, class T> __device__ __forceinline__ bool verify_ok(const P<T>& L, const P<T>& R, const U<T>& F, T dt, const Phys& ph) {

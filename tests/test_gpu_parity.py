@@ -108,6 +108,37 @@ def tiny_mhd(shape, bc):
     return (shape, tuple(1.0 / n for n in shape), cfg, [(bc, allf)], w, B, {"init_B": True})
 
 
+def random_hydro(shape, solver, ctu=1):
+    """Random hydro state, periodic on every face, the given Riemann solver id."""
+    g = P.ghosts_for(False, ctu=bool(ctu))
+    w, B = P.empty_state(shape, g, False)
+    rng = np.random.default_rng(5)
+    wi = P.interior(w, g, shape)
+    wi[..., 0] = 1.0 + 0.3 * rng.random(shape)
+    wi[..., 1] = 1.0 + 0.3 * rng.random(shape)
+    wi[..., 2:5] = 0.3 * (rng.random(shape + (3,)) - 0.5)
+    cfg = dict(mhd=0, gamma=5.0 / 3.0, solver=solver, limiter=2, ctu=ctu)
+    allf = {1: 3, 2: 15, 3: 63}[len(shape)]
+    return (shape, tuple(1.0 / n for n in shape), cfg, [(1, allf)], w, B, {})
+
+
+@pytest.mark.parametrize("solver", [0, 1, 2, 3, 4, 5])
+def test_solver_choice_hydro(solver):
+    """The runtime solver choice (Flux.cpp:34-45): HLLX and HLLC take HLLC, HLL
+    and HLLE themselves, and every other id -- the exact solver's and HLLD's,
+    which is MHD only -- the exact solver."""
+    run_pair(random_hydro((12, 10), solver), steps=4, nf=5)
+
+
+@pytest.mark.parametrize("solver", [0, 1, 2, 3, 4, 5])
+def test_solver_choice_mhd(solver):
+    """The runtime solver choice for MHD (Flux.cpp:46-53): HLL, HLLE, and HLLD
+    for every other id."""
+    pb = tiny_mhd((6, 5), 1)
+    pb[2]["solver"] = solver
+    run_pair(pb, steps=4)
+
+
 @pytest.mark.parametrize("shape", [(1, 1), (2, 3), (1, 1, 1), (2, 1, 3)])
 @pytest.mark.parametrize("bc", [0, 1])
 def test_tiny_grids(shape, bc):
