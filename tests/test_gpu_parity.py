@@ -108,16 +108,16 @@ def tiny_mhd(shape, bc):
     return (shape, tuple(1.0 / n for n in shape), cfg, [(bc, allf)], w, B, {"init_B": True})
 
 
-def random_hydro(shape, solver, ctu=1):
+def random_hydro(shape, solver, ctu=1, muscl=1):
     """Random hydro state, periodic on every face, the given Riemann solver id."""
-    g = P.ghosts_for(False, ctu=bool(ctu))
+    g = P.ghosts_for(False, ctu=bool(ctu), muscl=bool(muscl))
     w, B = P.empty_state(shape, g, False)
     rng = np.random.default_rng(5)
     wi = P.interior(w, g, shape)
     wi[..., 0] = 1.0 + 0.3 * rng.random(shape)
     wi[..., 1] = 1.0 + 0.3 * rng.random(shape)
     wi[..., 2:5] = 0.3 * (rng.random(shape + (3,)) - 0.5)
-    cfg = dict(mhd=0, gamma=5.0 / 3.0, solver=solver, limiter=2, ctu=ctu)
+    cfg = dict(mhd=0, gamma=5.0 / 3.0, solver=solver, limiter=2, ctu=ctu, muscl=muscl)
     allf = {1: 3, 2: 15, 3: 63}[len(shape)]
     return (shape, tuple(1.0 / n for n in shape), cfg, [(1, allf)], w, B, {})
 
@@ -137,6 +137,32 @@ def test_solver_choice_mhd(solver):
     pb = tiny_mhd((6, 5), 1)
     pb[2]["solver"] = solver
     run_pair(pb, steps=4)
+
+
+SWITCHES = ([("limiter", v) for v in range(5)] + [("cfl_mode", v) for v in (0, 1, 2)] +
+            [("ct_energy", v) for v in (0, 1, 2)])
+
+
+@pytest.mark.parametrize("key,value", SWITCHES)
+def test_switches_mhd(key, value):
+    """Each Config.h switch value on 2D MHD (CTU + HLLD + CT): the five limiters,
+    the CFL modes (max, add, power mean) and the three CT energy modes."""
+    pb = tiny_mhd((9, 7), 1)
+    pb[2][key] = value
+    run_pair(pb, steps=4)
+
+
+@pytest.mark.parametrize("key,value", [("limiter", v) for v in range(5)] + [("cfl_mode", v) for v in (0, 1, 2)] +
+                         [("muscl", 0), ("ctu", 0)])
+def test_switches_hydro(key, value):
+    """The same on 2D hydro (CTU + HLLC), plus first order (no MUSCL, no CTU)
+    and MUSCL without CTU."""
+    if key == "muscl":
+        pb = random_hydro((9, 7), 5, ctu=0, muscl=0)
+    else:
+        pb = random_hydro((9, 7), 5)
+        pb[2][key] = value
+    run_pair(pb, steps=4, nf=5)
 
 
 @pytest.mark.parametrize("shape", [(1, 1), (2, 3), (1, 1, 1), (2, 1, 3)])
