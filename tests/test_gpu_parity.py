@@ -165,6 +165,38 @@ def test_switches_hydro(key, value):
     run_pair(pb, steps=4, nf=5)
 
 
+def with_bc(pb, kind):
+    faces = {1: 3, 2: 15, 3: 63}[len(pb[0])]
+    return (pb[0], pb[1], pb[2], [(kind, faces)], pb[4], pb[5], pb[6])
+
+
+@pytest.mark.parametrize("kind", [0, 2])
+@pytest.mark.parametrize("case", ["mhd3", "mhd2", "hydro3"])
+def test_steps_with_boundaries(case, kind):
+    """Whole steps under outflow (0) and reflective (2) boundaries on every face
+    (the configurations run periodic or outflow): ghost values feed the MUSCL
+    stencils, the CTU corrections and the corner EMFs at the box edges."""
+    pb = {"mhd3": lambda: tiny_mhd((6, 5, 7), 1), "mhd2": lambda: tiny_mhd((9, 7), 1),
+          "hydro3": lambda: random_hydro((6, 5, 7), 5)}[case]()
+    run_pair(with_bc(pb, kind), steps=3, nf=5 if case.startswith("hydro") else 8)
+
+
+def test_mhd_1d():
+    """1D MHD (cell-centred B, no CT), MUSCL + HLLD, outflow."""
+    g = P.ghosts_for(True, ctu=False)
+    shape = (16,)
+    w, B = P.empty_state(shape, g, True)
+    rng = np.random.default_rng(9)
+    wi = P.interior(w, g, shape)
+    wi[..., 0] = 1 + 0.3 * rng.random(shape)
+    wi[..., 1] = 1 + 0.3 * rng.random(shape)
+    wi[..., 2:5] = 0.3 * (rng.random(shape + (3,)) - 0.5)
+    wi[..., 5] = 0.5
+    wi[..., 6] = 0.3 * rng.random(shape)
+    wi[..., 7] = 0.2
+    run_pair((shape, (1 / 16,), dict(mhd=1, gamma=5 / 3, solver=5, limiter=2, ctu=0), [(0, 3)], w, B, {}), steps=3)
+
+
 @pytest.mark.parametrize("shape", [(1, 1), (2, 3), (1, 1, 1), (2, 1, 3)])
 @pytest.mark.parametrize("bc", [0, 1])
 def test_tiny_grids(shape, bc):
