@@ -181,6 +181,24 @@ def test_steps_with_boundaries(case, kind):
     run_pair(with_bc(pb, kind), steps=3, nf=5 if case.startswith("hydro") else 8)
 
 
+@pytest.mark.parametrize("cfl", [0.3, 1.2])
+def test_mhd_double_rarefaction(cfl):
+    """2D MHD with two strong rarefactions (v = -+3 about x = 1/2, p = 0.02):
+    HLLD's physical-safety pivot to HLLE, and at CFL 1.2 rejected steps and
+    dt halving, against the oracle."""
+    g = P.ghosts_for(True)
+    shape = (32, 4)
+    w, B = P.empty_state(shape, g, True)
+    wi = P.interior(w, g, shape)
+    wi[..., 0] = 1.0
+    wi[..., 1] = 0.02
+    wi[:16, :, 2] = -3.0
+    wi[16:, :, 2] = 3.0
+    P.interior(B, g, shape, extra=1)[..., 0] = 0.05
+    cfg = dict(mhd=1, gamma=5 / 3, solver=5, limiter=2, ctu=1, ct_energy=2, cfl_coeff=cfl)
+    run_pair((shape, (1 / 32, 1 / 4), cfg, [(0, 15)], w, B, {"init_B": True}), steps=15)
+
+
 def test_mhd_1d():
     """1D MHD (cell-centred B, no CT), MUSCL + HLLD, outflow."""
     g = P.ghosts_for(True, ctu=False)
