@@ -659,50 +659,46 @@ __device__ __forceinline__ void sweep_march_body(const SweepArgs& a, long t) {
     if (D == 3) { idx[T0] = a.lo[T0] + (int)(r % n0); r /= n0; }
     const int f0 = a.lo[AX] + (int)r * a.chunk;
     const int f1 = min(a.hi[AX], f0 + a.chunk);
-    P<R> Lp, Rp, Lc, Rc;
+    P<R> Rp = pzero<R>(), Lc, Rc;
     idx[AX] = f0 - 1;
+    // One loop body rebuilds every cell's states, the walker's first cell (f0 -
+    // 1, no face of its own) included: with FMA contraction a second inlined
+    // copy may round differently, and a periodic image face rebuilt by the
+    // other copy would then take a different update than its twin.
+    const long s = a.ce.s[AX];
+    const int clo = -a.ce.g, chi = a.ce.n[AX] + a.ce.g;  // cells that exist along AX
+    long m = a.ce.at(idx[0], idx[1], idx[2]);
+    auto ld = [&](int c, long mm) { return c >= clo && c < chi ? load_p<double>(a.W, mm, MHD) : pzero<double>(); };
     if constexpr (WIN) {
         // The three cells of the MUSCL stencil stay in registers as the
         // walker advances; the cell two ahead is read (as FP64) before the
         // Riemann solve of the current face and narrowed when it is used.
-        const long s = a.ce.s[AX];
-        const int clo = -a.ce.g, chi = a.ce.n[AX] + a.ce.g;  // cells that exist along AX
-        long m = a.ce.at(idx[0], idx[1], idx[2]);
-        auto ld = [&](int c, long mm) { return c >= clo && c < chi ? load_p<double>(a.W, mm, MHD) : pzero<double>(); };
         P<R> wl = p_cast<R>(ld(f0 - 2, m - s)), wc = p_cast<R>(ld(f0 - 1, m));
-        P<R> wr = p_cast<R>(ld(f0, m + s));
-        cell_states_from<D, MHD, R, AX, FINAL>(a, idx[0], idx[1], idx[2], wl, wc, wr, Lp, Rp);
-        P<double> nxt = ld(f0 + 1, m + 2 * s);
-        for (int f = f0; f < f1; f++) {
+        P<double> nxt = ld(f0, m + s);
+        for (int f = f0 - 1; f < f1; f++) {
             idx[AX] = f;
-            m += s;
-            wl = wc;
-            wc = wr;
-            wr = p_cast<R>(nxt);
+            const P<R> wr = p_cast<R>(nxt);
             cell_states_from<D, MHD, R, AX, FINAL>(a, idx[0], idx[1], idx[2], wl, wc, wr, Lc, Rc);
             nxt = ld(f + 2, m + 2 * s);
-            face_flux<MHD, R, AX>(a, Rp, Lc, m);
+            if (f >= f0) face_flux<MHD, R, AX>(a, Rp, Lc, m);  // uniform per wavefront
             Rp = Rc;
+            wl = wc;
+            wc = wr;
+            m += s;
         }
     } else {
         // The cell ahead of the stencil is read before the Riemann solve of
         // the current face; the two behind it are read again (cache hits):
         // holding them too takes the FP64 walker to 256 registers.
-        const long s = a.ce.s[AX];
-        const int clo = -a.ce.g, chi = a.ce.n[AX] + a.ce.g;  // cells that exist along AX
-        long m = a.ce.at(idx[0], idx[1], idx[2]);
-        auto ld = [&](int c, long mm) { return c >= clo && c < chi ? load_p<double>(a.W, mm, MHD) : pzero<double>(); };
-        cell_states_from<D, MHD, R, AX, FINAL>(a, idx[0], idx[1], idx[2], p_cast<R>(ld(f0 - 2, m - s)), p_cast<R>(ld(f0 - 1, m)),
-                                               p_cast<R>(ld(f0, m + s)), Lp, Rp);
-        P<double> nxt = ld(f0 + 1, m + 2 * s);
-        for (int f = f0; f < f1; f++) {
+        P<double> nxt = ld(f0, m + s);
+        for (int f = f0 - 1; f < f1; f++) {
             idx[AX] = f;
-            m += s;
             cell_states_from<D, MHD, R, AX, FINAL>(a, idx[0], idx[1], idx[2], p_cast<R>(ld(f - 1, m - s)), p_cast<R>(ld(f, m)),
                                                    p_cast<R>(nxt), Lc, Rc);
             nxt = ld(f + 2, m + 2 * s);
-            face_flux<MHD, R, AX>(a, Rp, Lc, m);
+            if (f >= f0) face_flux<MHD, R, AX>(a, Rp, Lc, m);
             Rp = Rc;
+            m += s;
         }
     }
 }

@@ -65,3 +65,22 @@ def test_mixed_tiny_grids(shape):
     res, divb = run_mixed(tiny_mhd(shape, 1), 2)
     assert res["rel_l1"][0] <= 1e-5
     assert divb <= 1e-12
+
+
+@pytest.mark.parametrize("key,value", [("limiter", v) for v in range(5)] + [("ct_energy", v) for v in (0, 1, 2)])
+def test_mixed_switches_mhd(key, value):
+    """The mixed form under each limiter and CT energy mode (2D MHD)."""
+    from test_gpu_parity import tiny_mhd
+    pb = tiny_mhd((24, 20), 1)
+    pb[2][key] = value
+    res, divb = run_mixed(pb, 6)
+    assert res["rel_l1"][0] <= 1e-5
+    assert divb <= 1e-12
+
+
+@pytest.mark.parametrize("solver", [0, 1, 2, 4])
+def test_mixed_solvers_hydro(solver):
+    """The mixed form's exact, HLL, HLLC and HLLE solvers (2D hydro)."""
+    from test_gpu_parity import random_hydro
+    res, _ = run_mixed(random_hydro((24, 20), solver), 6, nf=5)
+    assert res["rel_l1"][0] <= 1e-5
