@@ -161,3 +161,33 @@ def test_batch_failed_patch_reports_and_keeps_state():
             G.advance(1e9, True, 1)
         assert b.stats(p)["steps"] == nsteps
         assert np.array_equal(b.get_state(p)[0], G.get_w()), p
+
+
+@pytest.mark.parametrize("limiter", [0, 4])
+def test_batch_mixed_periodic_div_b(limiter):
+    """Mixed batch, periodic patches, minmod and van Albada, 20 steps: the
+    periodic image faces take identical updates, so normalised max|div B|
+    stays at round-off."""
+    import dragon_amd as D
+    pbs = [P.rotor_2d(nx, B0, r0, u0, ny=ny) for nx, ny, B0, r0, u0 in PATCHES]
+    cfg = D.Config(precision=1)
+    for k, v in pbs[0][2].items():
+        setattr(cfg, k, v)
+    cfg.limiter = limiter
+    b = D.Batch([pb[0] for pb in pbs], [pb[1] for pb in pbs], cfg, bc="periodic")
+    for p, pb in enumerate(pbs):
+        b.set_state(p, pb[4], pb[5])
+    b.initialize_B_fields()
+    b.run(20)
+    b.sync()
+    worst = 0.0
+    for p, pb in enumerate(pbs):
+        w, B = b.get_state(p)
+        g = b.g
+        nx, ny = pb[0]
+        Bx = B[g:g + nx + 1, g:g + ny, 0]
+        By = B[g:g + nx, g:g + ny + 1, 1]
+        div = (Bx[1:, :] - Bx[:-1, :]) / pb[1][0] + (By[:, 1:] - By[:, :-1]) / pb[1][1]
+        worst = max(worst, np.abs(div).max() * min(pb[1]) / np.abs(B[..., :2]).max())
+    print(f"mixed batch periodic limiter {limiter}: normalised max|div B| {worst:.3g}", flush=True)
+    assert worst <= 1e-12
