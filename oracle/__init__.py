@@ -109,6 +109,9 @@ def _load():
         "oc_ct_faraday": (None, [cp] + [C.c_int] * 5 + [_dp, C.c_double, C.c_double, C.c_double, C.c_int, _dp]),
         "oc_ct_body_fields": (None, [cp] + [C.c_int] * 5 + [_dp, _dp]),
         "oc_cfl_speed": (C.c_double, [cp, _dp, C.c_double, C.c_double, C.c_double]),
+        "oc_correct_state": (None, [cp] + [C.c_int] * 5 + [_dp, _dp, _dp, C.c_double, C.c_int, C.c_int]),
+        "oc_ctu_flux": (C.c_int, [cp] + [C.c_int] * 5 + [_dp, _dp, _dp, _dp, C.c_double, C.c_double, C.c_int, C.c_char_p,
+                                                       C.c_int]),
     }
     for n, (r, a) in sig.items():
         f = getattr(lib, n)
@@ -378,6 +381,24 @@ def ct_body_fields(cfg: Cfg, shp, g, B, W):
     lib().oc_ct_body_fields(C.byref(c), d, nx, ny, nz, g, _ptr(B), _ptr(W))
 
 
+
+def correct_state(cfg: Cfg, shp, g, L, R, F, dt_dL, sdim, fdim):
+    """correctState (CTU.cpp:264-301): L and R updated in place.  2D takes the
+    flux direction fdim only (sdim is ignored)."""
+    c = cfg.c()
+    d, nx, ny, nz = _shape5(shp)
+    lib().oc_correct_state(C.byref(c), d, nx, ny, nz, g, _ptr(L), _ptr(R), _ptr(F), dt_dL, sdim, fdim)
+
+
+def ctu_flux(cfg: Cfg, axis, shp, g, L, R, T, F, dt_d, dt_t, dimt):
+    """computeCTUFlux_X/Y/Z (CTU.cpp:306-370): F updated in place."""
+    c = cfg.c()
+    _, nx, ny, nz = _shape5(shp)
+    err = C.create_string_buffer(256)
+    if lib().oc_ctu_flux(C.byref(c), axis, nx, ny, nz, g, _ptr(L), _ptr(R), _ptr(T), _ptr(F), dt_d, dt_t, dimt, err, 256):
+        raise OracleError(err.value.decode())
+
+
 # ------------------------------------------------------------ reference
 REF_DIR = os.path.join(HERE, "_ref")
 
@@ -415,6 +436,13 @@ class RefLib:
                 "rf_faraday3D": (None, [C.c_int] * 4 + [_dp, C.c_double, C.c_double, C.c_double, C.c_int, _dp]),
                 "rf_body_fields2D": (None, [C.c_int] * 3 + [_dp, _dp]),
                 "rf_body_fields3D": (None, [C.c_int] * 4 + [_dp, _dp]),
+            })
+        else:
+            sig.update({
+                "rf_correct_state2D": (None, [C.c_int] * 3 + [_dp, _dp, _dp, C.c_double, C.c_int]),
+                "rf_correct_state3D": (None, [C.c_int] * 4 + [_dp, _dp, _dp, C.c_double, C.c_int, C.c_int]),
+                "rf_ctu_flux3D": (C.c_int, [C.c_int] * 5 + [_dp, _dp, _dp, _dp, C.c_double, C.c_double, C.c_int,
+                                                             C.c_char_p, C.c_int]),
             })
         for n, (r, a) in sig.items():
             f = getattr(L, n)

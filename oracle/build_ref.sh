@@ -8,8 +8,11 @@
 #   hydro    Examples/1D/Sod/Config.h (hydro), TESTMODE
 #   mhdprod  DRAGON/Config.h (MHD), production switches (CT beta-gated, MC, HLLD)
 #   mhdg14   Examples/MHDRotor/{Config.h,Constants.h} (gamma = 1.4), TESTMODE
-# Unsplit.cpp / Splitting.cpp / CTU.cpp are not built: the first two include
-# <format>, which this image's C++ library does not provide.
+# Unsplit.cpp and Splitting.cpp are not built: they include <format>, which
+# this image's C++ library does not provide.  CTU.cpp is built into the hydro
+# variant for correctState and computeCTUFlux_X/Y/Z, which need neither; its
+# ctu_sweep_* functions call Unsplit.cpp and are dropped at link time
+# (-ffunction-sections, --gc-sections: nothing references them).
 set -euo pipefail
 REF=${REF:-/root/reference}
 HERE=$(cd "$(dirname "$0")" && pwd)
@@ -32,11 +35,12 @@ FLAGS="-std=c++23 -O2 -fPIC -ffp-contract=off -fno-fast-math -ffunction-sections
 build() {
   local name=$1; shift
   local pre="$1"; shift
+  local extra="$1"; shift
   local defs="$*"
   local objs=""
   local tmp=$OUT/obj_$name
   mkdir -p "$tmp"
-  for s in $SRCS; do
+  for s in $SRCS $extra; do
     local o=$tmp/$(basename "$s" .cpp).o
     $CXX $FLAGS $defs $pre $INC -c "$REF/$s" -o "$o"
     objs="$objs $o"
@@ -46,7 +50,7 @@ build() {
   rm -rf "$tmp"
   echo "built $OUT/libref_$name.so"
 }
-build mhd "" -DTESTMODE
-build hydro "-I$REF/Examples/1D/Sod" -DTESTMODE
-build mhdprod ""
-build mhdg14 "-I$REF/Examples/MHDRotor" -DTESTMODE
+build mhd "" "" -DTESTMODE
+build hydro "-I$REF/Examples/1D/Sod" "DRAGON/Hydro/Godunov/CTU.cpp" -DTESTMODE
+build mhdprod "" ""
+build mhdg14 "-I$REF/Examples/MHDRotor" "" -DTESTMODE

@@ -216,6 +216,22 @@ void oc_ct_body_fields(const oc_cfg* c, int dim, int nx, int ny, int nz, int g, 
     CFG = to_cfg(c);
     test_body_fields(dim, nx, ny, nz, g, B, W);
 }
+void oc_correct_state(const oc_cfg* c, int dim, int nx, int ny, int nz, int g, double* L, double* R, const double* F,
+                      double dt_dL, int sdim, int fdim) {
+    CFG = to_cfg(c);
+    test_correct_state(dim, nx, ny, nz, g, L, R, F, dt_dL, sdim, fdim);
+}
+int oc_ctu_flux(const oc_cfg* c, int axis, int nx, int ny, int nz, int g, const double* L, const double* R, const double* T,
+                double* F, double dt_d, double dt_t, int dimt, char* err, int n) {
+    CFG = to_cfg(c);
+    try {
+        test_ctu_flux(axis, nx, ny, nz, g, L, R, T, F, dt_d, dt_t, dimt);
+        return 0;
+    } catch (const std::exception& e) {
+        if (err && n > 0) { std::strncpy(err, e.what(), n - 1); err[n - 1] = 0; }
+        return 1;
+    }
+}
 double oc_cfl_speed(const oc_cfg* c, const double* w8, double dx, double dy, double dz) {
     CFG = to_cfg(c);
     return test_cfl_speed(P8(w8), dx, dy, dz);

@@ -1186,6 +1186,32 @@ void test_body_fields(int dim, int nx, int ny, int nz, int g, const double* B, d
         compute_body_fields3D(b, w); save_arr(w, W);
     }
 }
+// correctState (CTU.cpp:264-301) on loaded arrays; L, R updated in place.
+void test_correct_state(int dim, int nx, int ny, int nz, int g, double* L, double* R, const double* F, double dt_dL,
+                        int sdim, int fdim) {
+    if (dim == 2) {
+        Arr<Prim> l(nx, ny, g), r(nx, ny, g);
+        Arr<Cons> f(nx, ny, g);
+        load_arr(l, L); load_arr(r, R); load_arr(f, F);
+        correct_state2D(l, r, f, dt_dL, fdim);
+        save_arr(l, L); save_arr(r, R);
+    } else {
+        Arr<Prim> l(nx, ny, nz, g), r(nx, ny, nz, g);
+        Arr<Cons> f(nx, ny, nz, g);
+        load_arr(l, L); load_arr(r, R); load_arr(f, F);
+        correct_state3D(l, r, f, dt_dL, sdim, fdim);
+        save_arr(l, L); save_arr(r, R);
+    }
+}
+// computeCTUFlux_X/Y/Z (CTU.cpp:306-370) on loaded arrays; F updated in place.
+void test_ctu_flux(int axis, int nx, int ny, int nz, int g, const double* L, const double* R, const double* T, double* F,
+                   double dt_d, double dt_t, int dimt) {
+    Arr<Prim> l(nx, ny, nz, g), r(nx, ny, nz, g);
+    Arr<Cons> t(nx, ny, nz, g), f(nx, ny, nz, g);
+    load_arr(l, L); load_arr(r, R); load_arr(t, T); load_arr(f, F);
+    ctu_flux3D(l, r, t, f, dt_d, dt_t, axis, dimt);
+    save_arr(f, F);
+}
 }  // namespace oracle
 namespace oracle {
 double test_cfl_speed(const Prim& W, double dx, double dy, double dz) { return cfl_speed(W, dx, dy, dz); }

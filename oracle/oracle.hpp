@@ -219,6 +219,10 @@ void test_electric(int dim, int nx, int ny, int nz, int g, const double* FX, con
 void test_body_electric(int dim, int nx, int ny, int nz, int g, const double* W, int gg, double* E);
 void test_faraday(int dim, int nx, int ny, int nz, int g, const double* E, double dx, double dy, double dz, int gg, double* B);
 void test_body_fields(int dim, int nx, int ny, int nz, int g, const double* B, double* W);
+void test_correct_state(int dim, int nx, int ny, int nz, int g, double* L, double* R, const double* F, double dt_dL,
+                        int sdim, int fdim);
+void test_ctu_flux(int axis, int nx, int ny, int nz, int g, const double* L, const double* R, const double* T, double* F,
+                   double dt_d, double dt_t, int dimt);
 }  // namespace oracle
 namespace oracle {
 double test_cfl_speed(const Prim& W, double dx, double dy, double dz);

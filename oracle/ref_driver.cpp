@@ -9,7 +9,11 @@
 //
 // The reference's step orchestration (Hydro/Godunov/Unsplit.cpp,
 // Splitting.cpp) includes <format>, which this image's C++ library lacks, so
-// whole-step trajectories cannot be produced from the reference here.
+// whole-step trajectories cannot be produced from the reference here.  The
+// hydro build also compiles Hydro/Godunov/CTU.cpp, whose correctState and
+// computeCTUFlux_X/Y/Z (the 3D 12-solve corner fluxes) need nothing from
+// those two files; ctu_sweep_hydro / ctu_sweep_MHD call Unsplit.cpp's
+// computeFlux_* and applyFluxes and are not exported (the linker drops them).
 #include <cstring>
 #include <stdexcept>
 
@@ -21,6 +25,9 @@
 #include "Riemann.hpp"
 #include "TVD.hpp"
 #include "Constants.h"
+#ifndef MHD
+#include "Unsplit.hpp"
+#endif
 
 #define API extern "C" __attribute__((visibility("default")))
 
@@ -155,18 +162,7 @@ API void rf_speeds(const double* w8, double* out) {
 API int rf_is_physical(const double* w8) { return P8(w8).isPhysical() ? 1 : 0; }
 API double rf_cfl_speed(const double* w8, double dx, double dy, double dz) { return CFL::cfl_speed(P8(w8), dx, dy, dz); }
 
-#ifdef MHD
-// ---- CT pieces on small 2D/3D arrays in ExtendedArray layout
-static void load(MagneticArray2D& A, const double* a) {
-    int g = A.getGhosts(), m = 0;
-    for (int i = -g; i < A.getSizeX() + g; i++)
-        for (int j = -g; j < A.getSizeY() + g; j++, m += 3) A[i, j] = vec3(a[m], a[m + 1], a[m + 2]);
-}
-static void save(const MagneticArray2D& A, double* a) {
-    int g = A.getGhosts(), m = 0;
-    for (int i = -g; i < A.getSizeX() + g; i++)
-        for (int j = -g; j < A.getSizeY() + g; j++, m += 3) { a[m] = A[i, j].x; a[m + 1] = A[i, j].y; a[m + 2] = A[i, j].z; }
-}
+// ---- small 2D/3D arrays in ExtendedArray layout (ghosts included, last index fastest)
 static void load(FluxArray2D& A, const double* a) {
     int g = A.getGhosts(), m = 0;
     for (int i = -g; i < A.getSizeX() + g; i++)
@@ -182,23 +178,17 @@ static void save(const FluidArray2D& A, double* a) {
     for (int i = -g; i < A.getSizeX() + g; i++)
         for (int j = -g; j < A.getSizeY() + g; j++, m += 8) W8(A[i, j], a + m);
 }
-static void load(MagneticArray3D& A, const double* a) {
-    int g = A.getGhosts(), m = 0;
-    for (int i = -g; i < A.getSizeX() + g; i++)
-        for (int j = -g; j < A.getSizeY() + g; j++)
-            for (int k = -g; k < A.getSizeZ() + g; k++, m += 3) A[i, j, k] = vec3(a[m], a[m + 1], a[m + 2]);
-}
-static void save(const MagneticArray3D& A, double* a) {
-    int g = A.getGhosts(), m = 0;
-    for (int i = -g; i < A.getSizeX() + g; i++)
-        for (int j = -g; j < A.getSizeY() + g; j++)
-            for (int k = -g; k < A.getSizeZ() + g; k++, m += 3) { a[m] = A[i, j, k].x; a[m + 1] = A[i, j, k].y; a[m + 2] = A[i, j, k].z; }
-}
 static void load(FluxArray3D& A, const double* a) {
     int g = A.getGhosts(), m = 0;
     for (int i = -g; i < A.getSizeX() + g; i++)
         for (int j = -g; j < A.getSizeY() + g; j++)
             for (int k = -g; k < A.getSizeZ() + g; k++, m += 8) A[i, j, k] = C8(a + m);
+}
+static void save(const FluxArray3D& A, double* a) {
+    int g = A.getGhosts(), m = 0;
+    for (int i = -g; i < A.getSizeX() + g; i++)
+        for (int j = -g; j < A.getSizeY() + g; j++)
+            for (int k = -g; k < A.getSizeZ() + g; k++, m += 8) U8(A[i, j, k], a + m);
 }
 static void load(FluidArray3D& A, const double* a) {
     int g = A.getGhosts(), m = 0;
@@ -213,6 +203,29 @@ static void save(const FluidArray3D& A, double* a) {
             for (int k = -g; k < A.getSizeZ() + g; k++, m += 8) W8(A[i, j, k], a + m);
 }
 
+#ifdef MHD
+static void load(MagneticArray2D& A, const double* a) {
+    int g = A.getGhosts(), m = 0;
+    for (int i = -g; i < A.getSizeX() + g; i++)
+        for (int j = -g; j < A.getSizeY() + g; j++, m += 3) A[i, j] = vec3(a[m], a[m + 1], a[m + 2]);
+}
+static void save(const MagneticArray2D& A, double* a) {
+    int g = A.getGhosts(), m = 0;
+    for (int i = -g; i < A.getSizeX() + g; i++)
+        for (int j = -g; j < A.getSizeY() + g; j++, m += 3) { a[m] = A[i, j].x; a[m + 1] = A[i, j].y; a[m + 2] = A[i, j].z; }
+}
+static void load(MagneticArray3D& A, const double* a) {
+    int g = A.getGhosts(), m = 0;
+    for (int i = -g; i < A.getSizeX() + g; i++)
+        for (int j = -g; j < A.getSizeY() + g; j++)
+            for (int k = -g; k < A.getSizeZ() + g; k++, m += 3) A[i, j, k] = vec3(a[m], a[m + 1], a[m + 2]);
+}
+static void save(const MagneticArray3D& A, double* a) {
+    int g = A.getGhosts(), m = 0;
+    for (int i = -g; i < A.getSizeX() + g; i++)
+        for (int j = -g; j < A.getSizeY() + g; j++)
+            for (int k = -g; k < A.getSizeZ() + g; k++, m += 3) { a[m] = A[i, j, k].x; a[m + 1] = A[i, j, k].y; a[m + 2] = A[i, j, k].z; }
+}
 // E = computeElectric(FX, FY[, FZ], gg) then, if ref != NULL,
 // upwindElectric(E, FX, FY[, FZ], ref, gg).  Cell arrays are (n, g); edge
 // arrays (n+1, g).
@@ -285,5 +298,38 @@ API void rf_body_fields3D(int nx, int ny, int nz, int g, const double* B, double
     load(w, W);
     CT::computeBodyFields(b, w);
     save(w, W);
+}
+#endif
+
+#ifndef MHD
+// ---- CTU.cpp: correctState 2D/3D and the 12-solve corner fluxes
+// L, R: half-state arrays (n, g); F, T: flux arrays (n, g), all in place.
+API void rf_correct_state2D(int nx, int ny, int g, double* L, double* R, const double* F, double dt_dL, int dim) {
+    FluidArray2D l(nx, ny, g), r(nx, ny, g);
+    FluxArray2D f(nx, ny, g);
+    load(l, L); load(r, R); load(f, F);
+    correctState(l, r, f, dt_dL, dim);
+    save(l, L); save(r, R);
+}
+API void rf_correct_state3D(int nx, int ny, int nz, int g, double* L, double* R, const double* F, double dt_dL, int sdim, int fdim) {
+    FluidArray3D l(nx, ny, nz, g), r(nx, ny, nz, g);
+    FluxArray3D f(nx, ny, nz, g);
+    load(l, L); load(r, R); load(f, F);
+    correctState(l, r, f, dt_dL, sdim, fdim);
+    save(l, L); save(r, R);
+}
+// computeCTUFlux_{X,Y,Z}(L, R, T, F, dt_d, dt_t, dimt) for axis 0/1/2.
+API int rf_ctu_flux3D(int axis, int nx, int ny, int nz, int g, const double* L, const double* R, const double* T,
+                      double* F, double dt_d, double dt_t, int dimt, char* err, int n) {
+    try {
+        FluidArray3D l(nx, ny, nz, g), r(nx, ny, nz, g);
+        FluxArray3D t(nx, ny, nz, g), f(nx, ny, nz, g);
+        load(l, L); load(r, R); load(t, T); load(f, F);
+        if (axis == 0) computeCTUFlux_X(l, r, t, f, dt_d, dt_t, dimt);
+        else if (axis == 1) computeCTUFlux_Y(l, r, t, f, dt_d, dt_t, dimt);
+        else computeCTUFlux_Z(l, r, t, f, dt_d, dt_t, dimt);
+        save(f, F);
+        return 0;
+    } catch (const std::exception& e) { put_err(err, n, e.what()); return 1; }
 }
 #endif

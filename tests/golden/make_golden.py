@@ -10,6 +10,13 @@ function family; tests/test_oracle_golden.py checks the oracle against them
 without needing the reference.  Variants: "mhd" / "hydro" (TESTMODE, runtime
 solver/limiter switches), "mhdprod" (production switches: beta-gated CT
 energy), "mhdg14" (gamma = 1.4).
+
+golden_ctu.npz: Hydro/Godunov/CTU.cpp's correctState (2D, 3D) and
+computeCTUFlux_X/Y/Z (the 3D 12-solve corner fluxes), hydro build, HLLC and
+the exact solver.  golden_zerob.npz: Riemann fluxes, MUSCL-Hancock half
+states and CFL speeds on MHD states whose field is exactly zero on one or both
+sides, or below the HLLD floor (|B|^2 < 1e-100), in a near-uniform flow.
+Both use their own seeds, so the older fixtures are unchanged.
 """
 import os
 import sys
@@ -176,5 +183,110 @@ def main():
     print("wrote", path, os.path.getsize(path), "bytes")
 
 
+def ctu_cases(ref, rng):
+    """correctState 2D/3D and computeCTUFlux_X/Y/Z on small arrays (2 ghosts)."""
+    import ctypes as C
+    dp = C.POINTER(C.c_double)
+    P = lambda a: a.ctypes.data_as(dp)  # noqa: E731
+    g = 2
+
+    def states(shape):
+        n = int(np.prod(shape))
+        w = np.zeros((n, 8))
+        w[:, 0] = rng.uniform(0.5, 2.0, n)
+        w[:, 1] = rng.uniform(0.5, 2.0, n)
+        w[:, 2:5] = rng.normal(0, 0.5, (n, 3))
+        return w.reshape(shape + (8,))
+
+    def fluxes(shape):
+        f = rng.normal(0, 0.2, shape + (8,))
+        f[..., 5:] = 0.0
+        return f
+
+    out = {}
+    shp = (5, 4)
+    cells = tuple(s + 2 * g for s in shp)
+    for fdim in (0, 1):
+        L, R, F = states(cells), states(cells), fluxes(cells)
+        out[f"cs2_{fdim}_in"] = np.stack([L, R, F])
+        ref.lib.rf_correct_state2D(*shp, g, P(L), P(R), P(F), 0.13, fdim)
+        out[f"cs2_{fdim}_out"] = np.stack([L, R])
+    shp = (3, 2, 4)
+    cells = tuple(s + 2 * g for s in shp)
+    for sdim in range(3):
+        for fdim in range(3):
+            if sdim == fdim:
+                continue
+            L, R, F = states(cells), states(cells), fluxes(cells)
+            out[f"cs3_{sdim}{fdim}_in"] = np.stack([L, R, F])
+            ref.lib.rf_correct_state3D(*shp, g, P(L), P(R), P(F), 0.11, sdim, fdim)
+            out[f"cs3_{sdim}{fdim}_out"] = np.stack([L, R])
+    for solver in (O.HLLX, O.EXACT):
+        ref.set(riemann=solver, limiter=O.MINMOD)
+        for axis in range(3):
+            for dimt in range(3):
+                if dimt == axis:
+                    continue
+                L, R, T = states(cells), states(cells), fluxes(cells)
+                F = fluxes(cells) * 5.0
+                out[f"ctu_{solver}_{axis}{dimt}_in"] = np.stack([L, R, T, F])
+                err = C.create_string_buffer(256)
+                rc = ref.lib.rf_ctu_flux3D(axis, *shp, g, P(L), P(R), P(T), P(F), 0.2, 0.2 / 3.0, dimt, err, 256)
+                assert rc == 0, err.value
+                out[f"ctu_{solver}_{axis}{dimt}_out"] = F
+    return out
+
+
+def zerob_cases(ref, rng, n=150):
+    """MHD states in a near-uniform flow whose field is exactly zero on one or
+    both sides, tiny (~1e-6, the field-loop amplitude) or below the HLLD
+    zero-field floor (|B|^2 < 1e-100)."""
+    def states(kind):
+        rho = 1.0 + 0.01 * rng.normal(size=n)
+        p = 5.0 + 0.01 * rng.normal(size=n)
+        v = np.tile([2.0, 0.5, 0.0], (n, 1)) + 1e-3 * rng.normal(size=(n, 3))
+        if kind == "zero":
+            B = np.zeros((n, 3))
+        elif kind == "tiny":
+            B = 1e-6 * rng.normal(size=(n, 3))
+            B[::3, 0] = 0.0
+        else:  # below the floor
+            B = 1e-52 * rng.normal(size=(n, 3))
+        return np.column_stack([rho, p, v, B])
+
+    out = {}
+    kinds = [("zero", "zero"), ("tiny", "zero"), ("zero", "tiny"), ("tiny", "tiny"), ("sub", "sub"), ("sub", "zero")]
+    for a, b in kinds:
+        L, R, Rn = states(a), states(b), states(a)
+        key = f"{a}_{b}"
+        out[f"{key}_L"], out[f"{key}_R"], out[f"{key}_Rn"] = L, R, Rn
+        d = np.arange(n) % 3
+        for s in (O.HLLX, O.HLL, O.HLLE):
+            ref.set(riemann=s)
+            out[f"{key}_F_{s}"] = np.array([ref.riemann_flux(L[q], R[q], 0.1, int(d[q])) for q in range(n)])
+        for lim in range(5):
+            ref.set(limiter=lim)
+            LR = [ref.muscl(L[q], R[q], Rn[q], 0.05, R[q, 5:8] - L[q, 5:8]) for q in range(n)]
+            out[f"{key}_M_{lim}"] = np.array([np.concatenate(x) for x in LR])
+        ref.set()
+        out[f"{key}_cfl"] = np.array([[ref.cfl_speed(w, 0.1, 0.2, 0.0), ref.cfl_speed(w, 0.1, 0.2, 0.3)] for w in L])
+    return out
+
+
+def main_new():
+    ref = O.RefLib("hydro")
+    data = ctu_cases(ref, np.random.default_rng(SEED + 11))
+    path = os.path.join(HERE, "golden_ctu.npz")
+    np.savez_compressed(path, **data)
+    print("wrote", path, os.path.getsize(path), "bytes")
+    ref = O.RefLib("mhd")
+    data = zerob_cases(ref, np.random.default_rng(SEED + 12))
+    path = os.path.join(HERE, "golden_zerob.npz")
+    np.savez_compressed(path, **data)
+    print("wrote", path, os.path.getsize(path), "bytes")
+
+
 if __name__ == "__main__":
-    main()
+    if "--new-only" not in sys.argv:
+        main()
+    main_new()

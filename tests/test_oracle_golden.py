@@ -109,3 +109,63 @@ def test_body_fields_beta_gated_production_golden():
     W2 = d["W"].copy()
     O.ct_body_fields(O.Cfg(mhd=1, ct_energy=O.CT_BETA_GATED), (6, 5), 2, d["B"], W2)
     assert np.array_equal(W2, d["Wout"])
+
+
+def test_ctu_correct_state_and_corner_fluxes_golden():
+    """correctState 2D / 3D (every state and flux direction) and the 3D
+    12-solve corner fluxes computeCTUFlux_X/Y/Z (every transverse direction,
+    HLLC and the exact solver), written by the reference's own
+    Hydro/Godunov/CTU.cpp:264-370 (hydro build, 2 ghosts)."""
+    d = load("ctu")
+    g = 2
+    cfg = O.Cfg(mhd=0, solver=O.HLLX, limiter=O.MINMOD)
+    for fdim in (0, 1):
+        L, R, F = (a.copy() for a in d[f"cs2_{fdim}_in"])
+        O.correct_state(cfg, (5, 4), g, L, R, F, 0.13, 0, fdim)
+        Lr, Rr = d[f"cs2_{fdim}_out"]
+        assert np.array_equal(L, Lr) and np.array_equal(R, Rr), fdim
+    shp = (3, 2, 4)
+    for sdim in range(3):
+        for fdim in range(3):
+            if sdim == fdim:
+                continue
+            L, R, F = (a.copy() for a in d[f"cs3_{sdim}{fdim}_in"])
+            O.correct_state(cfg, shp, g, L, R, F, 0.11, sdim, fdim)
+            Lr, Rr = d[f"cs3_{sdim}{fdim}_out"]
+            assert np.array_equal(L, Lr) and np.array_equal(R, Rr), (sdim, fdim)
+    for solver in (O.HLLX, O.EXACT):
+        c = O.Cfg(mhd=0, solver=solver, limiter=O.MINMOD)
+        for axis in range(3):
+            for dimt in range(3):
+                if dimt == axis:
+                    continue
+                L, R, T, F = (a.copy() for a in d[f"ctu_{solver}_{axis}{dimt}_in"])
+                O.ctu_flux(c, axis, shp, g, L, R, T, F, 0.2, 0.2 / 3.0, dimt)
+                assert np.array_equal(F, d[f"ctu_{solver}_{axis}{dimt}_out"]), (solver, axis, dimt)
+
+
+ZB_KINDS = [("zero", "zero"), ("tiny", "zero"), ("zero", "tiny"), ("tiny", "tiny"), ("sub", "sub"), ("sub", "zero")]
+
+
+@pytest.mark.parametrize("a,b", ZB_KINDS)
+def test_zero_field_states_golden(a, b):
+    """MHD states whose field is exactly zero on one or both sides, ~1e-6
+    (the field-loop amplitude, with zero normal components) or below the HLLD
+    zero-field floor of 1e-100 (HLLD.cpp:50-51): Riemann fluxes for HLLD, HLL
+    and HLLE, MUSCL-Hancock half states for the five limiters, and CFL
+    speeds, against the reference's own code."""
+    d = load("zerob")
+    key = f"{a}_{b}"
+    L, R, Rn = d[f"{key}_L"], d[f"{key}_R"], d[f"{key}_Rn"]
+    n = len(L)
+    for s in (O.HLLX, O.HLL, O.HLLE):
+        cfg = O.Cfg(mhd=1, solver=s)
+        got = np.array([O.riemann_flux(cfg, L[q], R[q], 0.1, q % 3) for q in range(n)])
+        assert np.array_equal(got, d[f"{key}_F_{s}"]), s
+    for lim in range(5):
+        cfg = O.Cfg(mhd=1, limiter=lim)
+        got = np.array([np.concatenate(O.muscl(cfg, L[q], R[q], Rn[q], 0.05, R[q, 5:8] - L[q, 5:8])) for q in range(n)])
+        assert np.array_equal(got, d[f"{key}_M_{lim}"]), lim
+    cfg = O.Cfg(mhd=1)
+    got = np.array([[O.cfl_speed(cfg, w, 0.1, 0.2, 0.0), O.cfl_speed(cfg, w, 0.1, 0.2, 0.3)] for w in L])
+    assert np.array_equal(got, d[f"{key}_cfl"])
