@@ -74,6 +74,7 @@ class _Stats(C.Structure):
 
 
 _dp = C.POINTER(C.c_double)
+_BC_CALLBACK = C.CFUNCTYPE(C.c_int, C.c_void_p)
 _lib = None
 
 
@@ -98,6 +99,7 @@ def lib():
             "dg_set_boundary": (C.c_int, [vp, C.POINTER(_BC), C.c_int]),
             "dg_apply_boundary": (C.c_int, [vp]),
             "dg_apply_boundary_element": (C.c_int, [vp, C.POINTER(_BC)]),
+            "dg_set_boundary_callback": (C.c_int, [vp, vp, vp]),
             "dg_cfl_dt": (C.c_int, [vp, _dp, C.POINTER(_Status)]),
             "dg_step": (C.c_int, [vp, C.c_double, C.POINTER(_Status)]),
             "dg_advance": (C.c_int, [vp, C.c_double, C.c_int, C.c_long, C.POINTER(_Stats), C.POINTER(_Status)]),
@@ -109,6 +111,12 @@ def lib():
             "dg_divb": (C.c_int, [vp, _dp]),
             "dg_timing": (C.c_int, [vp, C.c_int, _dp, C.POINTER(C.c_long)]),
             "dg_kernel_time": (C.c_int, [vp, C.c_int, C.POINTER(C.c_char_p), _dp, C.POINTER(C.c_long)]),
+            "dg_snapshot": (C.c_int, [vp, C.c_int, vp, vp]),
+            "dg_snapshot_wait": (C.c_int, [vp]),
+            "dg_restore": (C.c_int, [vp, C.c_int, vp, vp]),
+            "dg_set_spacing": (C.c_int, [vp, C.c_double, C.c_double, C.c_double]),
+            "dg_host_alloc": (C.c_int, [C.c_long, C.POINTER(vp)]),
+            "dg_host_free": (C.c_int, [vp]),
             "dg_batch_create": (C.c_int, [C.POINTER(_Desc), C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int), _dp, _dp, C.c_int,
                                           C.POINTER(vp)]),
             "dg_batch_destroy": (C.c_int, [vp]),
@@ -390,23 +398,54 @@ class Grid:
         return any(e.on_host for e in self._boundary.elements)
 
     def _push_bc(self):
-        # with host elements the device keeps a list of Ignore on every face, so
-        # the library's own boundary passes (dg_init_B, the split step) do nothing
-        # and apply_boundary() sequences the list itself
-        els = [Ignore(X | Y | Z)._c()] if self._host_bc() else [e._c() for e in self._boundary.elements]
-        key = [(b.kind, b.faces, b.corners, b.gated, tuple(b.state)) for b in els]
+        # with host elements the device list is one Ignore on every face and a
+        # boundary callback is set: every boundary pass of the library (the
+        # advance loops, the pass before each split sweep, dg_init_B) calls
+        # back here, and _apply_list sequences the list
+        host = self._host_bc()
+        els = [Ignore(X | Y | Z)._c()] if host else [e._c() for e in self._boundary.elements]
+        key = (host, [(b.kind, b.faces, b.corners, b.gated, tuple(b.state)) for b in els])
         if key == self._bc_pushed:
             return
         arr = (_BC * max(1, len(els)))(*els)
         self._chk(lib().dg_set_boundary(self._h, arr, len(els)))
+        if host:
+            self._cb = _BC_CALLBACK(self._bc_callback)
+            self._chk(lib().dg_set_boundary_callback(self._h, C.cast(self._cb, C.c_void_p), None))
+        else:
+            self._chk(lib().dg_set_boundary_callback(self._h, None, None))
+            self._cb = None
         self._bc_pushed = key
+
+    def _bc_callback(self, user):
+        try:
+            self._apply_list()
+            return 0
+        except DragonError as e:
+            self._cb_error = e
+            return e.code or DG_ERR_ARG
+        except Exception as e:  # a user GhostFill raised: reported after the library call returns
+            self._cb_error = e
+            return DG_ERR_ARG
+
+    def _raise_cb(self):
+        e, self._cb_error = getattr(self, "_cb_error", None), None
+        if e is not None:
+            raise e
 
     def apply_boundary(self):
         """BoundaryList::apply (BoundaryList.hpp:79-104)."""
         self._push_bc()
-        if not self._host_bc():
+        try:
             self._chk(lib().dg_apply_boundary(self._h))
-            return
+        finally:
+            self._raise_cb()
+
+    def _apply_list(self):
+        """The list with host elements, in the reference's order: the implicit
+        Outflow on uncovered faces, then each element; device elements run
+        through dg_apply_boundary_element, host ones on a downloaded copy that
+        is uploaded before the next device element (or at the end)."""
         els = self._boundary.elements
         covered = 0
         for e in els:
@@ -458,58 +497,35 @@ class Grid:
     def on_step_fail(self, e) -> bool:  # Unsplit.cpp:21-23 (root grid: retry)
         return True
 
-    def _advance_host_bc(self, dt, check_cfl, max_steps, step):
-        """The advance loop (CFL.cpp:160-238) run from the host, for boundary
-        lists with host-evaluated elements."""
-        steps = fails = 0
-        t_adv = 0.0
-        tol = self.config.timestep_tol
-        while dt > tol:
-            if 0 <= max_steps <= steps:
-                break
-            self.apply_boundary()
-            t1 = min(dt, self.cfl_time()) if check_cfl else dt
-            while True:
-                try:
-                    step(t1)
-                    break
-                except DragonError as e:
-                    if e.code in (DG_ERR_HIP, DG_ERR_ARG, DG_ERR_UNSUPPORTED) or not self.on_step_fail(e):
-                        raise
-                fails += 1
-                t1 *= 0.5
-                if t1 < tol:
-                    raise DragonError(DG_ERR_DT_UNDERFLOW, "Timestep has fallen below minimum. Exiting")
-            dt -= t1
-            t_adv += t1
-            steps += 1
-        self.t_advanced = t_adv
-        return steps, fails
-
     def advance_unsplit(self, dt: float, check_cfl=True, max_steps=-1):
         """Grid::advance_unsplit (CFL.cpp:213-238); returns (steps, failed attempts)."""
         self._push_bc()
-        if self._host_bc():
-            return self._advance_host_bc(dt, check_cfl, max_steps, self.unsplit_step)
         st, S = _Status(), _Stats()
-        self._chk(lib().dg_advance(self._h, float(dt), int(check_cfl), int(max_steps), C.byref(S), C.byref(st)), st)
+        try:
+            self._chk(lib().dg_advance(self._h, float(dt), int(check_cfl), int(max_steps), C.byref(S), C.byref(st)), st)
+        finally:
+            self._raise_cb()
         self.t_advanced = float(S.t_advanced)  # simulated time of this call
         return int(S.steps), int(S.failures)
 
     def split_step(self, dt: float):
         """Grid2D/3D::split_step (Splitting.cpp:118-216), hydro; raises
         DragonError and commits nothing on failure."""
+        self._push_bc()
         st = _Status()
-        self._chk(lib().dg_split_step(self._h, float(dt), C.byref(st)), st)
+        try:
+            self._chk(lib().dg_split_step(self._h, float(dt), C.byref(st)), st)
+        finally:
+            self._raise_cb()
 
     def advance_split(self, dt: float, check_cfl=True, max_steps=-1):
         """Grid::advance_split (CFL.cpp:160-184); returns (steps, failed attempts)."""
         self._push_bc()
-        if self._host_bc():
-            raise DragonError(DG_ERR_UNSUPPORTED, "the split step applies its boundaries between sweeps on the device; "
-                                                  "host-evaluated GhostFill elements need the unsplit step")
         st, S = _Status(), _Stats()
-        self._chk(lib().dg_advance_split(self._h, float(dt), int(check_cfl), int(max_steps), C.byref(S), C.byref(st)), st)
+        try:
+            self._chk(lib().dg_advance_split(self._h, float(dt), int(check_cfl), int(max_steps), C.byref(S), C.byref(st)), st)
+        finally:
+            self._raise_cb()
         self.t_advanced = float(S.t_advanced)
         return int(S.steps), int(S.failures)
 
@@ -523,9 +539,10 @@ class Grid:
         """initialize_B_fields (B_fields.cpp:117-149): the boundary pass, then
         the cell-centred B from the faces."""
         self._push_bc()
-        if self._host_bc():
-            self.apply_boundary()  # the device list is Ignore, so dg_init_B's own pass does nothing
-        self._chk(lib().dg_init_B(self._h))
+        try:
+            self._chk(lib().dg_init_B(self._h))
+        finally:
+            self._raise_cb()
 
     def timing(self, enable: int = -1):
         """Start (1) / stop (0) / read (-1) live HIP-event timing of the step

@@ -28,6 +28,8 @@ struct dg_grid {
     Phys ph;
     hipStream_t s = nullptr;
     std::vector<dg_bc> bcs;
+    int (*bc_cb)(void* user) = nullptr;  // boundary callback (dg_set_boundary_callback)
+    void* bc_user = nullptr;
     double *W = nullptr, *Wn = nullptr;
     double *B = nullptr, *Bn = nullptr, *Bh = nullptr;
     double *HS = nullptr, *F = nullptr, *Fp = nullptr, *T = nullptr;
@@ -55,6 +57,13 @@ struct dg_grid {
     int n_ev = 0;
     double kms[K_COUNT] = {};
     long klaunch[K_COUNT] = {};
+    // frame snapshots (dg_snapshot): device staging in file order, a copy
+    // stream for the device-to-host transfer, events ordering the two
+    double* snap = nullptr;
+    long snap_len = 0;
+    hipStream_t cs = nullptr;
+    hipEvent_t snap_ready = nullptr, snap_done = nullptr;
+    bool snap_recorded = false;  // snap_done has been recorded (set by dg_snapshot only)
 };
 
 static const unsigned long long NO_ERR = ~0ull;
@@ -228,7 +237,8 @@ int dg_destroy(dg_grid* g) {
     if (!g) return DG_OK;
     int rc = DG_OK;
     if (g->s && hipStreamSynchronize(g->s) != hipSuccess) rc = DG_ERR_HIP;
-    double* bufs[] = {g->W, g->Wn, g->B, g->Bn, g->Bh, g->HS, g->F, g->Fp, g->T, g->E0, g->Eh, g->cflpart, g->Ws};
+    if (g->cs && hipStreamSynchronize(g->cs) != hipSuccess) rc = DG_ERR_HIP;
+    double* bufs[] = {g->W, g->Wn, g->B, g->Bn, g->Bh, g->HS, g->F, g->Fp, g->T, g->E0, g->Eh, g->cflpart, g->Ws, g->snap};
     for (double* p : bufs)
         if (p && hipFree(p) != hipSuccess) rc = DG_ERR_HIP;
     if (g->dev_words && hipFree(g->dev_words) != hipSuccess) rc = DG_ERR_HIP;
@@ -237,6 +247,9 @@ int dg_destroy(dg_grid* g) {
     if (g->ev1 && hipEventDestroy(g->ev1) != hipSuccess) rc = DG_ERR_HIP;
     for (hipEvent_t e : g->evs)
         if (hipEventDestroy(e) != hipSuccess) rc = DG_ERR_HIP;
+    if (g->snap_ready && hipEventDestroy(g->snap_ready) != hipSuccess) rc = DG_ERR_HIP;
+    if (g->snap_done && hipEventDestroy(g->snap_done) != hipSuccess) rc = DG_ERR_HIP;
+    if (g->cs && hipStreamDestroy(g->cs) != hipSuccess) rc = DG_ERR_HIP;
     if (g->s && hipStreamDestroy(g->s) != hipSuccess) rc = DG_ERR_HIP;
     delete g;
     return rc;
@@ -387,8 +400,20 @@ int dg_apply_boundary_element(dg_grid* g, const dg_bc* bc) {
     return DG_OK;
 }
 
+int dg_set_boundary_callback(dg_grid* g, int (*cb)(void* user), void* user) {
+    if (!g) return DG_ERR_ARG;
+    g->bc_cb = cb;
+    g->bc_user = user;
+    return DG_OK;
+}
+
 int dg_apply_boundary(dg_grid* g) {
     if (!g) return DG_ERR_ARG;
+    if (g->bc_cb) {  // the caller applies the list itself (host-evaluated elements)
+        g->cfl_cached = false;
+        const int rc = g->bc_cb(g->bc_user);
+        return rc ? fail(g, rc, "boundary callback failed") : DG_OK;
+    }
     int uncovered = 63;  // BoundaryList.hpp:104-110
     for (auto& b : g->bcs) uncovered &= ~b.faces;
     dg_bc imp{};
@@ -661,6 +686,149 @@ int dg_divb(dg_grid* g, double* out) {
     *out = bm > 0 ? dv * dmin / bm : 0.0;
     return DG_OK;
 }
+
+}  // extern "C"
+
+// ================================================ frame snapshot / restore
+// The frame files store each field x fastest (HDF5Output.cpp:305-342); the
+// device arrays are last-index fastest.  One field of a box of A x Bm x C
+// points (a: x, b: y in 3D, c: the contiguous axis) is transposed through a
+// 64 x 64 tile in LDS, so the reads and the writes are both coalesced.
+struct XfastView {
+    long sa, sb, sc, off;  // device strides of a, b, c and the offset of the box origin
+    long A, Bm, C;         // box extents
+};
+__global__ __launch_bounds__(256) void k_to_xfast(const double* src, XfastView v, double* dst) {
+    __shared__ double tile[64][65];
+    const int tx = threadIdx.x & 63, ty = threadIdx.x >> 6;
+    const long a0 = (long)blockIdx.x * 64, c0 = (long)blockIdx.y * 64, b = blockIdx.z;
+    for (int r = ty; r < 64; r += 4) {
+        const long a = a0 + r, c = c0 + tx;
+        if (a < v.A && c < v.C) tile[r][tx] = src[v.off + a * v.sa + b * v.sb + c * v.sc];
+    }
+    __syncthreads();
+    for (int r = ty; r < 64; r += 4) {
+        const long c = c0 + r, a = a0 + tx;
+        if (a < v.A && c < v.C) dst[(c * v.Bm + b) * v.A + a] = tile[tx][r];
+    }
+}
+__global__ __launch_bounds__(256) void k_from_xfast(const double* src, XfastView v, double* dst) {
+    __shared__ double tile[64][65];
+    const int tx = threadIdx.x & 63, ty = threadIdx.x >> 6;
+    const long a0 = (long)blockIdx.x * 64, c0 = (long)blockIdx.y * 64, b = blockIdx.z;
+    for (int r = ty; r < 64; r += 4) {
+        const long c = c0 + r, a = a0 + tx;
+        if (a < v.A && c < v.C) tile[tx][r] = src[(c * v.Bm + b) * v.A + a];
+    }
+    __syncthreads();
+    for (int r = ty; r < 64; r += 4) {
+        const long a = a0 + r, c = c0 + tx;
+        if (a < v.A && c < v.C) dst[v.off + a * v.sa + b * v.sb + c * v.sc] = tile[r][tx];
+    }
+}
+// The box [-ng, n + ng) (cells) of an Ext, dims per the grid's dimension.
+static XfastView xfast_view(const Ext& e, int dim, int ng) {
+    XfastView v{};
+    v.off = e.at(-ng, dim >= 2 ? -ng : 0, dim >= 3 ? -ng : 0);
+    if (dim == 3) {
+        v.sa = e.s[0]; v.sb = e.s[1]; v.sc = e.s[2];
+        v.A = e.n[0] + 2 * ng; v.Bm = e.n[1] + 2 * ng; v.C = e.n[2] + 2 * ng;
+    } else if (dim == 2) {
+        v.sa = e.s[0]; v.sb = 0; v.sc = e.s[1];
+        v.A = e.n[0] + 2 * ng; v.Bm = 1; v.C = e.n[1] + 2 * ng;
+    } else {
+        v.sa = e.s[0]; v.sb = 0; v.sc = 0;
+        v.A = e.n[0] + 2 * ng; v.Bm = 1; v.C = 1;
+    }
+    return v;
+}
+static void launch_xfast(bool to, const double* src, const XfastView& v, double* dst, hipStream_t s) {
+    const dim3 grid((unsigned)((v.A + 63) / 64), (unsigned)((v.C + 63) / 64), (unsigned)v.Bm);
+    if (to) hipLaunchKernelGGL(k_to_xfast, grid, dim3(256), 0, s, src, v, dst);
+    else hipLaunchKernelGGL(k_from_xfast, grid, dim3(256), 0, s, src, v, dst);
+}
+
+extern "C" {
+
+int dg_snapshot(dg_grid* g, int ng, double* cells, double* faces) {
+    if (!g || !cells || ng < 0 || ng > g->g) return DG_ERR_ARG;
+    const int D = g->d.dim;
+    const bool fb = g->d.mhd && D >= 2;
+    const XfastView vc = xfast_view(g->ce, D, ng), vf = xfast_view(g->fe, D, ng);
+    const long N = vc.A * vc.Bm * vc.C, NF = fb ? vf.A * vf.Bm * vf.C : 0;
+    const long need = 8 * N + 3 * NF;
+    if (!g->cs) {
+        HIPCHK(hipStreamCreateWithFlags(&g->cs, hipStreamNonBlocking));
+        HIPCHK(hipEventCreateWithFlags(&g->snap_ready, hipEventDisableTiming));
+        HIPCHK(hipEventCreateWithFlags(&g->snap_done, hipEventDisableTiming));
+    }
+    if (g->snap_recorded) HIPCHK(hipStreamWaitEvent(g->s, g->snap_done, 0));  // staging may still be copied out
+    if (g->snap_len < need) {
+        HIPCHK(hipStreamSynchronize(g->s));
+        if (g->snap) HIPCHK(hipFree(g->snap));
+        g->snap = nullptr;
+        g->snap_len = 0;
+        HIPCHK(hipMalloc((void**)&g->snap, sizeof(double) * need));
+        g->snap_len = need;
+    }
+    const long nc = g->ce.size, nfc = g->fe.size;
+    for (int f = 0; f < 8; f++) launch_xfast(true, g->W + f * nc, vc, g->snap + f * N, g->s);
+    if (fb)
+        for (int c = 0; c < 3; c++) launch_xfast(true, g->B + c * nfc, vf, g->snap + 8 * N + c * NF, g->s);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipEventRecord(g->snap_ready, g->s));
+    HIPCHK(hipStreamWaitEvent(g->cs, g->snap_ready, 0));
+    HIPCHK(hipMemcpyAsync(cells, g->snap, sizeof(double) * 8 * N, hipMemcpyDeviceToHost, g->cs));
+    if (fb && faces) HIPCHK(hipMemcpyAsync(faces, g->snap + 8 * N, sizeof(double) * 3 * NF, hipMemcpyDeviceToHost, g->cs));
+    HIPCHK(hipEventRecord(g->snap_done, g->cs));
+    g->snap_recorded = true;
+    return DG_OK;
+}
+
+// Touches no handle state but the event, so it may run on another thread
+// while this one keeps stepping the grid.
+int dg_snapshot_wait(dg_grid* g) {
+    if (!g) return DG_ERR_ARG;
+    if (g->snap_done) HIPCHK(hipEventSynchronize(g->snap_done));
+    return DG_OK;
+}
+
+int dg_restore(dg_grid* g, int ng, const double* cells, const double* faces) {
+    if (!g || !cells || ng < 0 || ng > g->g) return DG_ERR_ARG;
+    const int D = g->d.dim;
+    const bool fb = g->d.mhd && D >= 2 && faces;
+    const XfastView vc = xfast_view(g->ce, D, ng), vf = xfast_view(g->fe, D, ng);
+    const long N = vc.A * vc.Bm * vc.C, NF = fb ? vf.A * vf.Bm * vf.C : 0;
+    if (int rc = dg_snapshot_wait(g)) return rc;
+    g->cfl_cached = false;
+    Staging stage;
+    HIPCHK(hipMalloc((void**)&stage.p, sizeof(double) * (8 * N + 3 * NF)));
+    HIPCHK(hipMemcpyAsync(stage.p, cells, sizeof(double) * 8 * N, hipMemcpyHostToDevice, g->s));
+    if (fb) HIPCHK(hipMemcpyAsync(stage.p + 8 * N, faces, sizeof(double) * 3 * NF, hipMemcpyHostToDevice, g->s));
+    const long nc = g->ce.size, nfc = g->fe.size;
+    for (int f = 0; f < g->nf; f++) launch_xfast(false, stage.p + f * N, vc, g->W + f * nc, g->s);
+    if (fb)
+        for (int c = 0; c < 3; c++) launch_xfast(false, stage.p + 8 * N + c * NF, vf, g->B + c * nfc, g->s);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(g->s));
+    HIPCHK(stage.release());
+    return DG_OK;
+}
+
+int dg_set_spacing(dg_grid* g, double dx, double dy, double dz) {
+    if (!g || !(dx > 0) || (g->d.dim >= 2 && !(dy > 0)) || (g->d.dim >= 3 && !(dz > 0))) return DG_ERR_ARG;
+    g->d.dx = dx;
+    if (g->d.dim >= 2) g->d.dy = dy;
+    if (g->d.dim >= 3) g->d.dz = dz;
+    g->cfl_cached = false;
+    return DG_OK;
+}
+
+int dg_host_alloc(long bytes, void** p) {
+    if (!p || bytes <= 0) return DG_ERR_ARG;
+    return hipHostMalloc(p, (size_t)bytes, hipHostMallocDefault) == hipSuccess ? DG_OK : DG_ERR_HIP;
+}
+int dg_host_free(void* p) { return !p || hipHostFree(p) == hipSuccess ? DG_OK : DG_ERR_HIP; }
 
 }  // extern "C"
 

@@ -117,6 +117,17 @@ int dg_download_device(dg_grid* g, double* w_soa, double* B_soa);
 int dg_set_boundary(dg_grid* g, const dg_bc* list, int n);
 /* boundary.apply(grid) (BoundaryList.hpp:84-93) */
 int dg_apply_boundary(dg_grid* g);
+/* A boundary list that holds user GhostFill subclasses evaluated on the host
+ * (Boundary/GhostFill.hpp:16-30): with a callback set, every boundary pass of
+ * the library -- dg_apply_boundary, the passes of dg_advance, and the pass
+ * before each directional sweep of dg_split_step (Splitting.cpp:219-299) --
+ * calls cb(user) instead of applying the device list.  The callback applies
+ * the whole list itself, in the reference's order (the implicit Outflow, then
+ * the elements), with dg_apply_boundary_element for built-in elements and
+ * dg_download / dg_upload around its own host edits; during a split step
+ * these act on the sweep's input state.  It returns 0 or a DG_* code.  NULL
+ * clears it. */
+int dg_set_boundary_callback(dg_grid* g, int (*cb)(void* user), void* user);
 /* One boundary element alone, without the implicit Outflow of the list (the
  * element's apply(), e.g. Outflow.cpp / Periodic.cpp): lets a caller keep the
  * reference's list order when some elements are user GhostFill subclasses
@@ -153,6 +164,31 @@ int dg_timing(dg_grid* g, int enable, double* step_ms, long* steps);
 int dg_kernel_time(dg_grid* g, int id, const char** name, double* ms, long* launches);
 /* Diagnostics: max over interior cells of |div B| * min(dx) / max|B_face| (0 for hydro / 1D). */
 int dg_divb(dg_grid* g, double* out);
+
+/* Frames and restarts (the reference's DRAGONHOARD; file I/O in dragon_hoard.h).
+ * dg_snapshot: the cells and faces that DRAGONHOARD::writeToFile reads
+ * (grid[i,j,k] and _B() over [-ng, n+ng), DRAGONHOARD/HDF5Output.cpp:305-342,
+ * 450-491), x fastest in the layout of dragon_hoard.h, copied into the host
+ * buffers asynchronously: a device transpose into a staging buffer on the
+ * grid's stream, then the device-to-host copy on a copy stream, so later
+ * steps overlap the transfer.  cells: 8 x dgh_cell_count doubles; faces: 3 x
+ * dgh_face_count (2D/3D MHD; may be NULL otherwise).  Host buffers from
+ * dg_host_alloc (pinned) let the copy run without a host-side staging pass;
+ * they must stay valid until dg_snapshot_wait returns.  A later snapshot waits
+ * for the previous copy.  0 <= ng <= dg_ghosts(g).  dg_snapshot_wait reads no
+ * handle state but the copy's event: it may be called from another thread
+ * while the caller keeps stepping the grid (a frame writer thread). */
+int dg_snapshot(dg_grid* g, int ng, double* cells, double* faces);
+int dg_snapshot_wait(dg_grid* g);
+/* DRAGONHOARD::loadFromFile's writes into the grid (HDF5Restart.cpp:205-261):
+ * the cells and faces of the box [-ng, n+ng) from buffers in the same layout;
+ * the rest of the state is kept.  Synchronous. */
+int dg_restore(dg_grid* g, int ng, const double* cells, const double* faces);
+/* grid.dx/dy/dz = ... as loadFromFile sets them (HDF5Restart.cpp:187-188, 290-292). */
+int dg_set_spacing(dg_grid* g, double dx, double dy, double dz);
+/* Pinned host memory for snapshot buffers (hipHostMalloc / hipHostFree). */
+int dg_host_alloc(long bytes, void** p);
+int dg_host_free(void* p);
 
 /* ---------------------------------------------------------------------
  * Batched patches.  Replaces the per-child dispatch of DistGrid<T>::step

@@ -14,10 +14,11 @@ from helpers import oracle_cfg
 
 pytestmark = pytest.mark.gpu
 
-# (nx, ny, B0, r0, u0): several size classes, non-square patches, x and y
-# faces beyond one wavefront
+# (nx, ny, B0, r0, u0): every size class (largest side < 64, < 128, < 192,
+# >= 192), non-square patches, x and y faces beyond one wavefront
 PATCHES = [(16, 16, 4.0, 0.1, 1.5), (24, 20, 3.0, 0.09, 1.2), (33, 17, 5.0, 0.12, 2.0), (40, 40, 2.5, 0.1, 1.0),
-           (12, 30, 4.5, 0.11, 1.8), (70, 66, 3.5, 0.1, 1.4), (20, 131, 4.0, 0.08, 1.6), (130, 24, 2.8, 0.12, 1.1)]
+           (12, 30, 4.5, 0.11, 1.8), (70, 66, 3.5, 0.1, 1.4), (20, 131, 4.0, 0.08, 1.6), (130, 24, 2.8, 0.12, 1.1),
+           (200, 24, 3.2, 0.1, 1.3)]
 
 
 def build(precision=0, patches=PATCHES, bc="outflow"):
@@ -33,9 +34,9 @@ def build(precision=0, patches=PATCHES, bc="outflow"):
     return b, pbs
 
 
-def oracle_patch(pb, bc="outflow"):
+def oracle_patch(pb, bc="outflow", threads=1):
     shape, d, cfg, bcs, w, B, meta = pb
-    G = O.Grid(shape, d, oracle_cfg(cfg), ghosts=1)
+    G = O.Grid(shape, d, oracle_cfg(cfg, threads), ghosts=1)
     if bc == "periodic":
         G.clear_bc()
         G.add_bc(1, O.FALL)
@@ -63,6 +64,28 @@ def test_batch_fp64_matches_oracle_per_patch():
               flush=True)
         assert st["steps"] == 5 and st["status"] == 0
         assert same_w and same_B
+
+
+def test_batch_fp64_100_steps_every_size_class():
+    """The drop-in bar on batched patches: 100 steps (60 eager, 40 replayed
+    from a HIP graph) of one patch per size class, the largest near the top
+    of the configuration's range (256 x 250), each bit-identical to the
+    oracle advancing that patch alone."""
+    patches = [(40, 36, 4.0, 0.1, 1.5), (100, 90, 3.0, 0.09, 1.2), (150, 64, 5.0, 0.12, 2.0), (256, 250, 2.6, 0.11, 1.7),
+               (197, 201, 4.4, 0.08, 1.1)]
+    b, pbs = build(0, patches)
+    b.run(60)
+    b.run(40, graph=True)
+    b.sync()
+    for p, pb in enumerate(pbs):
+        G = oracle_patch(pb, threads=8)
+        for _ in range(100):
+            G.advance(1e9, True, 1)
+        w, B = b.get_state(p)
+        st = b.stats(p)
+        same = np.array_equal(w, G.get_w()) and np.array_equal(B, G.get_B())
+        print(f"patch {pb[0]}: steps {st['steps']} fails {st['fails']} bit-equal {same}", flush=True)
+        assert st["steps"] == 100 and st["status"] == 0 and same
 
 
 def test_batch_periodic_matches_oracle_per_patch():

@@ -16,8 +16,8 @@ pytestmark = pytest.mark.gpu
 TOL = 1e-10
 
 
-def run_pair(problem, steps=None, t=None, nf=8):
-    og = oracle_grid(problem)
+def run_pair(problem, steps=None, t=None, nf=8, threads=8):
+    og = oracle_grid(problem, threads)
     gg = gpu_grid(problem)
     if t is not None:
         n_o = og.advance(t)
@@ -41,34 +41,97 @@ def test_c1_sod_1d_full():
     assert res["bit_equal"]
 
 
-def test_c2_kh_2d_small():
-    """C2 switch set (hydro CTU, HLLC, van Leer, periodic) at 64^2, 60 steps."""
-    run_pair(P.kelvin_helmholtz_2d(64), steps=60, nf=5)
+# The drop-in bar, <= 1e-10 relative L1 per variable after 100 steps, on every
+# configuration's switch set.
+def test_c2_kh_2d_100_steps():
+    """C2 switch set (hydro CTU, HLLC, van Leer, periodic) at 128^2, 100 steps."""
+    assert run_pair(P.kelvin_helmholtz_2d(128), steps=100, nf=5)["bit_equal"]
 
 
-def test_c3_orszag_tang_2d_small():
-    """C3 switch set (MHD CTU + CT, HLLD, MC, beta-gated) at 64^2, 40 steps."""
-    run_pair(P.orszag_tang_2d(64), steps=40)
+def test_c3_orszag_tang_2d_100_steps():
+    """C3 switch set (MHD CTU + CT, HLLD, MC, beta-gated) at 64^2, 100 steps."""
+    assert run_pair(P.orszag_tang_2d(64), steps=100)["bit_equal"]
 
 
-def test_c4_turbulence_3d_small():
-    """C4 switch set (3D MHD CTU + CT, HLLD, van Leer) at 16^3, 12 steps."""
-    run_pair(P.turbulence_3d(16), steps=12)
+def test_c4_turbulence_3d_100_steps():
+    """C4 switch set (3D MHD CTU + CT, HLLD, van Leer) at 32^3, 100 steps."""
+    assert run_pair(P.turbulence_3d(32), steps=100)["bit_equal"]
 
 
-def test_c5_rotor_patch():
-    """C5 switch set (MHD, gamma=1.4, outflow, minmod) on one 48^2 patch, 30 steps."""
-    run_pair(P.rotor_2d(48, B0=4.0, r0=0.1, u0=1.5), steps=30)
+def test_c5_rotor_patch_100_steps():
+    """C5 switch set (MHD, gamma=1.4, outflow, minmod) on one 64^2 patch, 100 steps."""
+    assert run_pair(P.rotor_2d(64, B0=4.0, r0=0.1, u0=1.5), steps=100)["bit_equal"]
 
 
-def test_hydro_3d_ctu12():
-    """3D hydro CTU 12-solve (diagonal advection) at 16^3, 10 steps."""
-    run_pair(P.diagonal_3d(16), steps=10, nf=5)
+def test_hydro_3d_ctu12_100_steps():
+    """3D hydro CTU 12-solve (diagonal advection, HLLC, MC) at 24^3, 100 steps."""
+    assert run_pair(P.diagonal_3d(24), steps=100, nf=5)["bit_equal"]
 
 
 def test_fp64_parity_100_steps_2d_mhd():
-    """The drop-in bar: <= 1e-10 relative L1 per variable after 100 steps."""
+    """2D MHD at a size that is not a multiple of the tile widths: 48^2, 100 steps."""
     run_pair(P.orszag_tang_2d(48), steps=100)
+
+
+# The reference's CTU integration tests (Testing/Core/Godunov/HydroCTU_Tests.cpp)
+# on the GPU, held to the oracle (tests/test_oracle_ctu.py runs their own
+# asserts on the oracle).  Test-build switches: minmod, CFL add, CT total energy.
+def ctu_case(shape, mhd, solver, kind):
+    g = P.ghosts_for(bool(mhd))
+    w, B = P.empty_state(shape, g, bool(mhd))
+    n = shape[0]
+    idx = np.meshgrid(*[np.arange(s) for s in shape], indexing="ij")
+    c = [(i + 0.5) / n for i in idx]
+    wi = P.interior(w, g, shape)
+    if kind == "diag":
+        wi[..., 0] = np.where((c[0] - c[1] > 0.0) if len(shape) == 2 else (c[0] - c[1] + 0.5 * c[2] > 0.25), 2.0, 1.0)
+        wi[..., 1] = 1.0
+        wi[..., 2:2 + len(shape)] = 1.0
+        bcs = [(1, {2: 15, 3: 63}[len(shape)])]
+    else:
+        r = np.sqrt(sum((x - 0.5) ** 2 for x in c))
+        wi[..., 0] = 1.0
+        wi[..., 1] = np.where(r < 0.1, 10.0, 0.1)
+        bcs = [(0, 63)]
+    cfg = dict(mhd=mhd, solver=solver, limiter=0, ctu=1, ct_energy=0, cfl_mode=1)
+    return (tuple(shape), tuple(1.0 / s for s in shape), cfg, bcs, w, B, {})
+
+
+# The hydro exact solver calls pow with non-integer exponents (Exact.cpp:34-159);
+# the device library's pow and glibc's may differ in the last bit, so its
+# cases are held to the 1e-10 bar (run_pair) and their bit equality printed.
+def exact_pow_bits(res, mhd):
+    if mhd:
+        assert res["bit_equal"]
+    else:
+        print("hydro exact solver: bit-equal" if res["bit_equal"] else f"hydro exact solver: max |diff| {res['max_abs']:.3g}")
+
+
+@pytest.mark.parametrize("mhd", [0, 1])
+def test_ctu_diagonal_contact_2d(mhd):
+    """HydroCTU_Tests.cpp:20-64: 40^2, 20 frames of 0.004 (exact solver; HLLD in MHD)."""
+    pb = ctu_case((40, 40), mhd, O.EXACT, "diag")
+    exact_pow_bits(run_pair(pb, t=0.08), mhd)
+
+
+@pytest.mark.parametrize("mhd", [0, 1])
+def test_ctu_diagonal_contact_3d(mhd):
+    """HydroCTU_Tests.cpp:66-112: 24^3 to t = 0.03 (HLLX)."""
+    pb = ctu_case((24, 24, 24), mhd, O.HLLX, "diag")
+    assert run_pair(pb, t=0.03)["bit_equal"]
+
+
+@pytest.mark.parametrize("mhd", [0, 1])
+def test_ctu_blast_2d(mhd):
+    """HydroCTU_Tests.cpp:153-212: 64^2 blast in an outflow box to t = 0.05."""
+    exact_pow_bits(run_pair(ctu_case((64, 64), mhd, O.EXACT, "blast"), t=0.05), mhd)
+
+
+@pytest.mark.parametrize("mhd,n", [(0, 64), (1, 32)])
+def test_ctu_blast_3d(mhd, n):
+    """HydroCTU_Tests.cpp:214-279: blast to t = 0.02 (64^3 hydro as the
+    reference runs it; 32^3 MHD)."""
+    assert run_pair(ctu_case((n, n, n), mhd, O.HLLX, "blast"), t=0.02, threads=16)["bit_equal"]
 
 
 def rect_mhd_3d(shape=(40, 9, 70)):
@@ -284,3 +347,19 @@ def test_verify_fallback_paths(name, dim, try_hlle):
     oracle, 1D and 2D CTU, with and without the HLLE stage."""
     res = run_pair(fallback_case(name, 200, dim, try_hlle), steps=40, nf=5)
     assert res["bit_equal"]
+
+
+# Switches read at dg_create from the environment (dg_api.hip): the
+# reference-structured unfused step (half-state arrays, separate CTU pass)
+# for MHD and 2D hydro, and a fixed walker length for the strided-axis sweeps.
+@pytest.mark.parametrize("make,steps,nf", [(lambda: P.orszag_tang_2d(40), 20, 8), (lambda: P.kelvin_helmholtz_2d(40), 20, 5),
+                                           (lambda: P.turbulence_3d(12), 8, 8)])
+def test_unfused_step_matches_oracle(monkeypatch, make, steps, nf):
+    monkeypatch.setenv("DG_UNFUSED", "1")
+    assert run_pair(make(), steps=steps, nf=nf)["bit_equal"]
+
+
+@pytest.mark.parametrize("chunk", ["3", "7", "1000"])
+def test_fixed_walker_length_matches_oracle(monkeypatch, chunk):
+    monkeypatch.setenv("DG_MARCH_CHUNK", chunk)
+    assert run_pair(rect_mhd_3d((20, 9, 17)), steps=5)["bit_equal"]
