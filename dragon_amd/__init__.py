@@ -131,6 +131,8 @@ def lib():
             "dg_batch_set_tiles": (C.c_int, [vp, C.POINTER(C.c_int), C.c_int]),
             "dg_batch_get_tiles": (C.c_int, [vp, C.POINTER(C.c_int), C.c_int]),
             "dg_batch_autotune": (C.c_int, [vp, C.c_int, C.POINTER(C.c_int), _dp]),
+            "dg_batch_set_ysweep": (C.c_int, [vp, C.POINTER(C.c_int), C.c_int]),
+            "dg_batch_get_ysweep": (C.c_int, [vp, C.POINTER(C.c_int), C.c_int]),
             "dg_batch_last_error": (C.c_char_p, [vp]),
             "dg_batch_stream": (vp, [vp]),
         }
@@ -693,8 +695,20 @@ class Batch:
         arr = (C.c_int * len(chunks))(*chunks)
         self._chk(lib().dg_batch_set_tiles(self._h, arr, len(chunks)))
 
+    def ysweep(self):
+        """The y sweep's segment width (lanes) per size class."""
+        out = (C.c_int * BATCH_CLASSES)()
+        self._chk(lib().dg_batch_get_ysweep(self._h, out, BATCH_CLASSES))
+        return list(out)
+
+    def set_ysweep(self, widths):
+        arr = (C.c_int * len(widths))(*widths)
+        self._chk(lib().dg_batch_set_ysweep(self._h, arr, len(widths)))
+
     def autotune(self, reps: int = 3):
-        """Pick the x-sweep tile per size class by timing; returns (tiles, ms per class launch)."""
+        """Pick the x-sweep tile and then the y-sweep segment width per size
+        class by timing; returns (tiles, ms per class launch); ysweep() reads
+        the widths chosen."""
         out = (C.c_int * BATCH_CLASSES)()
         ms = np.zeros(BATCH_CLASSES)
         self._chk(lib().dg_batch_autotune(self._h, int(reps), out, ms.ctypes.data_as(_dp)))

@@ -526,7 +526,9 @@ int dg_split_step(dg_grid* g, double dt, dg_status* st) {
         set_status(st, DG_ERR_UNSUPPORTED, 0, 0, 0, msg);
         return fail(g, DG_ERR_UNSUPPORTED, msg);
     }
-    if (!g->Ws && hipMalloc((void**)&g->Ws, sizeof(double) * g->nf * g->ce.size) != hipSuccess) return fail(g, DG_ERR_HIP, "hipMalloc");
+    // 8 fields like W and Wn: a boundary callback between sweeps sees Ws as
+    // the grid state, and dg_upload / dg_download move 8 fields
+    if (!g->Ws && hipMalloc((void**)&g->Ws, sizeof(double) * 8 * g->ce.size) != hipSuccess) return fail(g, DG_ERR_HIP, "hipMalloc");
     g->cfl_cached = false;
     launch_fill_u64(g->dev_words, NO_ERR, 1, g->s);
     static const int order3[6][3] = {{0, 1, 2}, {2, 0, 1}, {1, 2, 0}, {2, 1, 0}, {0, 2, 1}, {1, 0, 2}};  // Splitting.cpp:160-201
@@ -859,6 +861,7 @@ struct dg_batch {
     BlockTable bt[BK_COUNT] = {};
     long cells = 0, faces = 0, zones = 0, max_nc = 0, max_nfc = 0;
     int chunk[BATCH_CLASSES] = {8, 16, 32, 32};
+    int ysw[BATCH_CLASSES] = {64, 64, 64, 64};   // y-sweep segment width per class
     hipStream_t s = nullptr;
     hipGraph_t graph = nullptr;
     hipGraphExec_t gexec = nullptr;
@@ -888,6 +891,10 @@ static long batch_items(const dg_batch* b, const PatchDesc& q, int kind) {
         const long c = std::min<long>(b->chunk[q.cls], len);
         return ((len + c - 1) / c) * rows;
     };
+    auto segments = [&](long len, long rows) {  // whole wavefronts of SW-lane segments
+        const long sw = b->ysw[q.cls], segs = ((len + sw - 2) / (sw - 1)) * rows;
+        return (segs * sw + 63) / 64 * 64;
+    };
     switch (kind) {
         case BK_BCC_X: return sides * (ny + 6);
         case BK_BCC_Y: return 2 * (nx + 6) * 3;  // the contiguous axis: one item per ghost cell, both sides
@@ -895,11 +902,11 @@ static long batch_items(const dg_batch* b, const PatchDesc& q, int kind) {
         case BK_BCF_Y: return sides * (nx + 7);
         case BK_CELLS1: return (nx + 2) * (ny + 2);
         case BK_PRED_X: return chunked(nx + 3, ny + 4);
-        case BK_PRED_Y: return ((ny + 3 + 62) / 63) * (nx + 4) * 64;
+        case BK_PRED_Y: return segments(ny + 3, nx + 4);
         case BK_EDGES_P: return (nx + 3) * (ny + 3);
         case BK_FACES: return q.nfc;
         case BK_FLUX_X: return chunked(nx + 1, ny + 2);
-        case BK_FLUX_Y: return ((ny + 1 + 62) / 63) * (nx + 2) * 64;
+        case BK_FLUX_Y: return segments(ny + 1, nx + 2);
         case BK_EDGES: return (nx + 1) * (ny + 1);
         default: return q.nc;  // BK_CELLS
     }
@@ -943,7 +950,7 @@ static BatchArgs batch_args(const dg_batch* b, int commit, int cls_filter) {
     a.ph = b->ph;
     a.cfl_coeff = b->d.cfl_coeff; a.tol = b->d.timestep_tol > 0 ? b->d.timestep_tol : 1e-14;
     a.cfl_mode = b->d.cfl_mode; a.bc_kind = b->bc_kind;
-    for (int c = 0; c < BATCH_CLASSES; c++) a.chunk[c] = b->chunk[c];
+    for (int c = 0; c < BATCH_CLASSES; c++) { a.chunk[c] = b->chunk[c]; a.ysw[c] = b->ysw[c]; }
     a.commit = commit; a.cls_filter = cls_filter;
     return a;
 }
@@ -1167,33 +1174,68 @@ int dg_batch_get_tiles(const dg_batch* b, int* chunk, int n) {
 int dg_batch_autotune(dg_batch* b, int reps, int* chunk_out, double* class_ms) {
     if (!b || reps < 1) return DG_ERR_ARG;
     static const int cand[] = {2, 4, 8, 16, 32, 64, 128};
+    static const int ycand[] = {64, 32, 16};
     std::vector<int> ncls(BATCH_CLASSES, 0);
     for (const auto& q : b->P) ncls[q.cls]++;
     BCHK(hipStreamSynchronize(b->s));
     batch_drop_graph(b);
+    // one class's step (commit off: states and counters untouched), timed
+    auto time_class = [&](int c, float* ms) -> int {
+        batch_launch(b, 0, c, false);  // warm
+        BCHK(hipEventRecord(b->e0, b->s));
+        for (int r = 0; r < reps; r++) batch_launch(b, 0, c, false);
+        BCHK(hipEventRecord(b->e1, b->s));
+        BCHK(hipEventSynchronize(b->e1));
+        BCHK(hipEventElapsedTime(ms, b->e0, b->e1));
+        return DG_OK;
+    };
     for (int c = 0; c < BATCH_CLASSES; c++) {
         if (class_ms) class_ms[c] = 0;
         if (!ncls[c]) continue;
         double best = 1e300;
         int pick = b->chunk[c];
-        for (int k : cand) {
+        for (int k : cand) {  // x sweep: faces per walker
             b->chunk[c] = k;
             if (batch_table(b, BK_PRED_X) || batch_table(b, BK_FLUX_X)) return DG_ERR_HIP;
-            batch_launch(b, 0, c, false);  // warm; no commit: states and counters untouched
-            BCHK(hipEventRecord(b->e0, b->s));
-            for (int r = 0; r < reps; r++) batch_launch(b, 0, c, false);
-            BCHK(hipEventRecord(b->e1, b->s));
-            BCHK(hipEventSynchronize(b->e1));
             float ms = 0;
-            BCHK(hipEventElapsedTime(&ms, b->e0, b->e1));
+            if (int rc = time_class(c, &ms)) return rc;
             if (ms < best) { best = ms; pick = k; }
         }
         b->chunk[c] = pick;
+        if (batch_table(b, BK_PRED_X) || batch_table(b, BK_FLUX_X)) return DG_ERR_HIP;
+        int ypick = b->ysw[c];
+        best = 1e300;
+        for (int w : ycand) {  // y sweep: segment width
+            b->ysw[c] = w;
+            if (batch_table(b, BK_PRED_Y) || batch_table(b, BK_FLUX_Y)) return DG_ERR_HIP;
+            float ms = 0;
+            if (int rc = time_class(c, &ms)) return rc;
+            if (ms < best) { best = ms; ypick = w; }
+        }
+        b->ysw[c] = ypick;
         if (class_ms) class_ms[c] = best / reps;
     }
-    if (batch_table(b, BK_PRED_X) || batch_table(b, BK_FLUX_X)) return DG_ERR_HIP;
+    for (int kind : {BK_PRED_X, BK_FLUX_X, BK_PRED_Y, BK_FLUX_Y})
+        if (batch_table(b, kind)) return DG_ERR_HIP;
     if (chunk_out)
         for (int c = 0; c < BATCH_CLASSES; c++) chunk_out[c] = b->chunk[c];
+    return DG_OK;
+}
+
+int dg_batch_set_ysweep(dg_batch* b, const int* width, int n) {
+    if (!b || !width || n < 1 || n > BATCH_CLASSES) return DG_ERR_ARG;
+    for (int c = 0; c < n; c++)
+        if (width[c] != 64 && width[c] != 32 && width[c] != 16) return DG_ERR_ARG;
+    BCHK(hipStreamSynchronize(b->s));
+    batch_drop_graph(b);
+    for (int c = 0; c < n; c++) b->ysw[c] = width[c];
+    if (batch_table(b, BK_PRED_Y) || batch_table(b, BK_FLUX_Y)) return DG_ERR_HIP;
+    return DG_OK;
+}
+
+int dg_batch_get_ysweep(const dg_batch* b, int* width, int n) {
+    if (!b || !width || n < 1 || n > BATCH_CLASSES) return DG_ERR_ARG;
+    for (int c = 0; c < n; c++) width[c] = b->ysw[c];
     return DG_OK;
 }
 

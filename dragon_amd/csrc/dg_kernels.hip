@@ -718,30 +718,33 @@ template <class R> __device__ __forceinline__ P<R> shfl_up1(const P<R>& w) {
                 __shfl_up(w.vz, 1), __shfl_up(w.Bx, 1), __shfl_up(w.By, 1), __shfl_up(w.Bz, 1)};
 }
 
-// Contiguous axis: lane l of a wavefront rebuilds cell c0-1+l; lanes 1..63
-// take the left state from the lane below.
-template <int D, bool MHD, class R, int AX, bool FINAL>
-__device__ __forceinline__ void sweep_wave_body(const SweepArgs& a, long wave, int lane) {
+// Contiguous axis: lanes are grouped into segments of SW lanes (64, 32 or
+// 16); lane l of a segment rebuilds cell c0-1+l and lanes 1..SW-1 take the
+// left state from the lane below, so a segment computes SW-1 faces.  With
+// SW = 64 the segment is the wavefront and its index is wave-uniform.
+template <int D, bool MHD, class R, int AX, bool FINAL, int SW = 64>
+__device__ __forceinline__ void sweep_wave_body(const SweepArgs& a, long seg_id, int lane) {
     constexpr int T0 = 0;                 // slowest transverse dim
     constexpr int T1 = D == 3 ? 1 : 0;    // faster transverse dim
+    const int ll = lane % SW;             // lane within the segment
     const int len = a.hi[AX] - a.lo[AX];
-    const int nseg = (len + 62) / 63;
+    const int nseg = (len + SW - 2) / (SW - 1);
     const int n1 = D == 3 ? a.hi[T1] - a.lo[T1] : 1;
     const int n0 = a.hi[T0] - a.lo[T0];
-    if (wave >= (long)nseg * n0 * n1) return;  // uniform per wavefront
+    if (seg_id >= (long)nseg * n0 * n1) return;  // uniform per segment
     int idx[3] = {0, 0, 0};
-    int r = (int)wave;  // < 2^31 wavefronts: 32-bit division (the host sizes grids far below)
+    int r = (int)seg_id;  // < 2^31 segments: 32-bit division (the host sizes grids far below)
     const int seg = r % nseg;
     r /= nseg;
     if (D == 3) { idx[T1] = a.lo[T1] + r % n1; r /= n1; }
     idx[T0] = a.lo[T0] + r;
-    const int c = a.lo[AX] + seg * 63 - 1 + lane;
+    const int c = a.lo[AX] + seg * (SW - 1) - 1 + ll;
     const bool live = c < a.hi[AX];
     P<R> L = pzero<R>(), Rr = pzero<R>();
     idx[AX] = c;
     if (live) cell_states<D, MHD, R, AX, FINAL>(a, idx[0], idx[1], idx[2], L, Rr);
-    const P<R> Rl = shfl_up1(Rr);
-    if (lane >= 1 && live) face_flux<MHD, R, AX>(a, Rl, L, a.ce.at(idx[0], idx[1], idx[2]));
+    const P<R> Rl = shfl_up1(Rr);  // lane 0 of a segment takes the lane below's value and leaves it unused
+    if (ll >= 1 && live) face_flux<MHD, R, AX>(a, Rl, L, a.ce.at(idx[0], idx[1], idx[2]));
 }
 // Occupancy floor of the contiguous-axis sweeps: the FP64 ones need 176
 // registers, 8 short of three waves per SIMD; at three they spill 12 bytes
@@ -1805,9 +1808,13 @@ KS __attribute__((amdgpu_waves_per_eu(wave_min_waves<RT>()))) __global__ void kb
     BATCH_ITEM(bt)  // first is a multiple of the workgroup size: first / 64 + the wavefront in the workgroup
     const PatchView v = patch_view(A, p);
     SweepArgs s = batch_sweep_args<1, FIN>(A, v, p);
-    s.chunk = 63;
     const long w = (long)bt.first[blockIdx.x] / 64 + __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
-    sweep_wave_body<2, true, RT, 1, FIN>(s, w, threadIdx.x & 63);
+    const int lane = threadIdx.x & 63;
+    switch (A.ysw[v.cls]) {  // segment width of the patch's size class (uniform per workgroup)
+        case 16: s.chunk = 15; sweep_wave_body<2, true, RT, 1, FIN, 16>(s, w * 4 + lane / 16, lane); break;
+        case 32: s.chunk = 31; sweep_wave_body<2, true, RT, 1, FIN, 32>(s, w * 2 + lane / 32, lane); break;
+        default: s.chunk = 63; sweep_wave_body<2, true, RT, 1, FIN, 64>(s, w, lane); break;
+    }
 }
 template <bool PRED>
 KB __global__ void kb_emf(BatchArgs A, BlockTable bt) {

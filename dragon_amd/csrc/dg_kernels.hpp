@@ -113,6 +113,7 @@ struct BatchArgs {
     double cfl_coeff, tol;
     int cfl_mode, bc_kind;     // bc_kind: 0 outflow, 1 periodic (all faces, corner ghosts)
     int chunk[BATCH_CLASSES];  // x-sweep faces per thread, per size class
+    int ysw[BATCH_CLASSES];    // y-sweep segment width in lanes (64, 32, 16: 63, 31, 15 faces), per size class
     int commit;                // 0: run the step, keep every patch's state and counters (tuning)
     int cls_filter;            // -1: all patches, else only this size class
 };

@@ -108,7 +108,13 @@ long dg_face_count(const dg_grid* g);
  * stride = doubles per cell record (5 hydro, 8 MHD; more is allowed). B may be NULL. */
 int dg_upload(dg_grid* g, const double* w, int stride, const double* B);
 int dg_download(dg_grid* g, double* w, int stride, double* B);
-/* Device-resident state in/out, SoA (8 cell fields, 3 face fields); no host copy. */
+/* Device-resident state in/out, SoA (8 cell fields, 3 face fields); no host copy.
+ * Ordering: the copies run on the handle's own non-blocking stream (dg_stream),
+ * which does not wait for work on any other stream.  The caller must finish
+ * writing w_soa / B_soa (dg_upload_device), or reading them from an earlier
+ * dg_download_device, before the call: synchronise the producing stream, or
+ * enqueue that work on dg_stream(g).  Both calls return after the copies are
+ * complete (they synchronise the handle's stream). */
 int dg_upload_device(dg_grid* g, const double* w_soa, const double* B_soa);
 int dg_download_device(dg_grid* g, double* w_soa, double* B_soa);
 
@@ -225,12 +231,18 @@ int dg_batch_run(dg_batch* b, int nsteps, int use_graph, double* device_ms);
  * complete (st->i = patch index; the patch keeps its previous state). */
 int dg_batch_sync(dg_batch* b, dg_status* st);
 int dg_batch_patch_stats(dg_batch* b, int patch, double* dt_last, long* steps, long* fails, int* status);
-/* Size classes: largest side < 64, < 128, < 192, >= 192.  A tile is the
- * number of x faces one thread walks in the x sweep of a patch of that class.
- * autotune times each candidate on the class's patches (steps not committed)
- * and keeps the fastest; class_ms (optional) gets the best time per launch. */
+/* Size classes: largest side < 64, < 128, < 192, >= 192.  Two tile shapes
+ * per class: the number of x faces one thread walks in the x sweep (tiles),
+ * and the y sweep's segment width, 64, 32 or 16 lanes (one, two or four
+ * segments of 63, 31 or 15 faces per wavefront; narrow patches waste fewer
+ * lanes with short segments).  autotune times each candidate on the class's
+ * patches (steps not committed), the x tile first, then the y width, and
+ * keeps the fastest; class_ms (optional) gets the best time per launch.
+ * Results do not depend on either shape. */
 int dg_batch_set_tiles(dg_batch* b, const int* chunk, int n);
 int dg_batch_get_tiles(const dg_batch* b, int* chunk, int n);
+int dg_batch_set_ysweep(dg_batch* b, const int* width, int n);
+int dg_batch_get_ysweep(const dg_batch* b, int* width, int n);
 int dg_batch_autotune(dg_batch* b, int reps, int* chunk_out, double* class_ms);
 const char* dg_batch_last_error(const dg_batch* b);
 void* dg_batch_stream(dg_batch* b);

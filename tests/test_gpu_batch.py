@@ -118,6 +118,8 @@ def test_batch_graph_replays_match_eager():
 
 
 def test_batch_tiles_do_not_change_results():
+    """x-sweep tiles and y-sweep segment widths (set, or chosen by the
+    autotuner) leave every patch's results bit for bit unchanged."""
     a, _ = build(0)
     t = a.tiles()
     a.run(2)
@@ -126,13 +128,18 @@ def test_batch_tiles_do_not_change_results():
     tiles, ms = b.autotune(reps=1)
     print("autotuned tiles", tiles, "ms per class launch", ms, flush=True)
     assert all(1 <= x <= 4096 for x in tiles)
-    b.set_tiles([3, 5, 7, 9])
-    b.run(2)
-    b.sync()
-    for p in range(len(PATCHES)):
-        wa, Ba = a.get_state(p)
-        wb, Bb = b.get_state(p)
-        assert np.array_equal(wa, wb) and np.array_equal(Ba, Bb), p
+    print("autotuned y-sweep segment widths", b.ysweep(), flush=True)
+    assert all(w in (16, 32, 64) for w in b.ysweep())
+    for xt, yw in (([3, 5, 7, 9], [16, 32, 64, 16]), ([64, 2, 11, 4], [32, 16, 16, 64])):
+        c, _ = build(0)
+        c.set_tiles(xt)
+        c.set_ysweep(yw)
+        c.run(2)
+        c.sync()
+        for p in range(len(PATCHES)):
+            wa, Ba = a.get_state(p)
+            wc, Bc = c.get_state(p)
+            assert np.array_equal(wa, wc) and np.array_equal(Ba, Bc), (xt, yw, p)
 
 
 def test_batch_mixed_close_to_fp64():
