@@ -102,6 +102,10 @@ def lib():
             "dg_set_boundary_callback": (C.c_int, [vp, vp, vp]),
             "dg_cfl_dt": (C.c_int, [vp, _dp, C.POINTER(_Status)]),
             "dg_step": (C.c_int, [vp, C.c_double, C.POINTER(_Status)]),
+            "dg_step_noalloc": (C.c_int, [C.POINTER(_Desc), vp, vp, vp, vp, C.c_double, vp, C.POINTER(_Status)]),
+            "dg_noalloc_scratch_bytes": (C.c_long, [C.POINTER(_Desc)]),
+            "dg_noalloc_scratch_limit": (C.c_long, []),
+            "dg_noalloc_last_error": (C.c_char_p, []),
             "dg_advance": (C.c_int, [vp, C.c_double, C.c_int, C.c_long, C.POINTER(_Stats), C.POINTER(_Status)]),
             "dg_split_step": (C.c_int, [vp, C.c_double, C.POINTER(_Status)]),
             "dg_advance_split": (C.c_int, [vp, C.c_double, C.c_int, C.c_long, C.POINTER(_Stats), C.POINTER(_Status)]),

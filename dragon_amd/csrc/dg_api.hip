@@ -8,6 +8,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -823,6 +824,128 @@ int dg_set_spacing(dg_grid* g, double dx, double dy, double dz) {
     if (g->d.dim >= 2) g->d.dy = dy;
     if (g->d.dim >= 3) g->d.dz = dz;
     g->cfl_cached = false;
+    return DG_OK;
+}
+
+// ====================================================== no-allocation step
+// dg_step_noalloc: the unsplit step on caller-owned device buffers, with its
+// scratch (fluxes, half-step faces, EMFs, the failure word) in this
+// file-scope device array instead of allocations.  Each GPU holds its own
+// copy; its address is looked up once per device (hipGetSymbolAddress) and
+// only that address is handed to kernels and copies.
+static constexpr long NOALLOC_SCRATCH_BYTES = 64L << 20;
+static __device__ __align__(256) unsigned char scratch[NOALLOC_SCRATCH_BYTES];
+static constexpr int NOALLOC_MAX_DEVICES = 64;
+static std::atomic<void*> g_scratch_ptr[NOALLOC_MAX_DEVICES] = {};
+static thread_local std::string g_noalloc_err;
+
+// Scratch layout of one step (the arrays dg_create allocates for the same
+// grid, minus the state): doubles from the start of the array.
+struct NoallocLayout {
+    long words = 0, F = 0, Fp = -1, HS = -1, T = -1, Bh = -1, E0 = -1, Eh = -1, end = 0;
+};
+static NoallocLayout noalloc_layout(const dg_desc& d, const Ext& ce, const Ext& fe, bool fused) {
+    NoallocLayout L;
+    const long nc = ce.size, nfc = fe.size, nf = d.mhd ? 8 : 5;
+    long at = 4;  // failure key + 3 spare words (u64, the size of a double)
+    auto take = [&](long n) { const long o = at; at += (n + 31) / 32 * 32; return o; };
+    L.words = 0;
+    L.F = take((long)d.dim * nf * nc);
+    if (fused) { if (d.ctu) L.Fp = take((long)d.dim * nf * nc); }
+    else L.HS = take(2L * d.dim * nf * nc);
+    if (!d.mhd && d.dim == 3 && d.ctu) L.T = take(5L * nf * nc);
+    if (d.mhd && d.dim >= 2) { L.Bh = take(3 * nfc); L.E0 = take(3 * nfc); L.Eh = take(3 * nfc); }
+    L.end = at;
+    return L;
+}
+
+extern "C" long dg_noalloc_scratch_bytes(const dg_desc* desc) {
+    if (!desc || desc->dim < 1 || desc->dim > 3) return -1;
+    dg_desc d = *desc;
+    int gh = d.ghosts;
+    if (d.mhd && d.ctu) gh = std::max(gh, 3);
+    else if (d.muscl) gh = std::max(gh, 2);
+    else gh = std::max(gh, 1);
+    const Ext ce = make_ext(d.dim, d.nx, d.ny, d.nz, gh), fe = make_ext(d.dim, d.nx + 1, d.ny + 1, d.nz + 1, gh);
+    const bool fused = (d.mhd && d.dim >= 2) || (!d.mhd && d.dim == 2);
+    return noalloc_layout(d, ce, fe, fused).end * (long)sizeof(double);
+}
+extern "C" long dg_noalloc_scratch_limit(void) { return NOALLOC_SCRATCH_BYTES; }
+extern "C" const char* dg_noalloc_last_error(void) { return g_noalloc_err.c_str(); }
+
+static int noalloc_fail(int code, const std::string& m) {
+    g_noalloc_err = m;
+    return code;
+}
+
+extern "C" int dg_step_noalloc(const dg_desc* desc, const double* w_in, const double* B_in, double* w_out, double* B_out,
+                               double dt, void* stream, dg_status* st) {
+    if (!desc || !w_in || !w_out) return noalloc_fail(DG_ERR_ARG, "dg_step_noalloc: null argument");
+    const dg_desc& d = *desc;
+    if (d.dim < 1 || d.dim > 3 || d.nx < 1 || (d.dim >= 2 && d.ny < 1) || (d.dim >= 3 && d.nz < 1))
+        return noalloc_fail(DG_ERR_ARG, "dg_step_noalloc: bad sizes");
+    if (!(d.dx > 0) || (d.dim >= 2 && !(d.dy > 0)) || (d.dim >= 3 && !(d.dz > 0)) || (d.ctu && !d.muscl))
+        return noalloc_fail(DG_ERR_ARG, "dg_step_noalloc: bad spacing or switches");
+    const bool faces = d.mhd && d.dim >= 2;
+    if (faces && (!B_in || !B_out)) return noalloc_fail(DG_ERR_ARG, "dg_step_noalloc: face B buffers required for 2D/3D MHD");
+    if (w_in == w_out || (faces && B_in == B_out)) return noalloc_fail(DG_ERR_ARG, "dg_step_noalloc: outputs alias the inputs");
+    const long need = dg_noalloc_scratch_bytes(desc);
+    if (need > NOALLOC_SCRATCH_BYTES)
+        return noalloc_fail(DG_ERR_ARG, "dg_step_noalloc: the grid needs " + std::to_string(need) + " bytes of scratch, more than the " +
+                                            std::to_string(NOALLOC_SCRATCH_BYTES) + "-byte static array");
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= NOALLOC_MAX_DEVICES)
+        return noalloc_fail(DG_ERR_HIP, "dg_step_noalloc: no current device");
+    void* sp = g_scratch_ptr[dev].load(std::memory_order_acquire);
+    if (!sp) {  // once per device, with that device current (a racing first call stores the same address)
+        const hipError_t e = hipGetSymbolAddress(&sp, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !sp) return noalloc_fail(DG_ERR_HIP, std::string("hipGetSymbolAddress: ") + hipGetErrorString(e));
+        g_scratch_ptr[dev].store(sp, std::memory_order_release);
+    }
+    double* base = (double*)sp;
+    // a dg_grid on the stack describing the caller's buffers; nothing allocated
+    dg_grid g;
+    g.d = d;
+    if (g.d.timestep_tol <= 0) g.d.timestep_tol = 1e-14;
+    int gh = d.ghosts;
+    if (d.mhd && d.ctu) gh = std::max(gh, 3);
+    else if (d.muscl) gh = std::max(gh, 2);
+    else gh = std::max(gh, 1);
+    g.g = gh;
+    g.nf = d.mhd ? 8 : 5;
+    g.ce = make_ext(d.dim, d.nx, d.ny, d.nz, gh);
+    g.fe = make_ext(d.dim, d.nx + 1, d.ny + 1, d.nz + 1, gh);
+    g.ph = make_phys(d);
+    g.fused = (d.mhd && d.dim >= 2) || (!d.mhd && d.dim == 2);
+    const NoallocLayout L = noalloc_layout(d, g.ce, g.fe, g.fused);
+    g.W = const_cast<double*>(w_in);
+    g.Wn = w_out;
+    if (faces) { g.B = const_cast<double*>(B_in); g.Bn = B_out; }
+    g.F = base + L.F;
+    if (L.Fp >= 0) g.Fp = base + L.Fp;
+    if (L.HS >= 0) g.HS = base + L.HS;
+    if (L.T >= 0) g.T = base + L.T;
+    if (L.Bh >= 0) { g.Bh = base + L.Bh; g.E0 = base + L.E0; g.Eh = base + L.Eh; }
+    g.dev_words = (unsigned long long*)(base + L.words);
+    g.cflpart = nullptr;
+    g.s = (hipStream_t)stream;
+    // the failure word is the only scratch that must start at a set value
+    launch_fill_u64(g.dev_words, NO_ERR, 1, g.s);
+    StepCtx c = make_ctx(&g, dt);
+    if (c.mixed) p32::launch_step(c, g.s);
+    else p64::launch_step(c, g.s);
+    hipError_t e = hipGetLastError();
+    unsigned long long key = NO_ERR;
+    if (e == hipSuccess) e = hipMemcpyAsync(&key, g.dev_words, sizeof key, hipMemcpyDeviceToHost, g.s);
+    if (e == hipSuccess) e = hipStreamSynchronize(g.s);
+    if (e != hipSuccess) return noalloc_fail(DG_ERR_HIP, std::string("dg_step_noalloc: ") + hipGetErrorString(e));
+    if (key != NO_ERR) {
+        int code;
+        std::string msg;
+        decode(&g, key, st, &code, &msg);
+        return noalloc_fail(code, msg);
+    }
+    set_status(st, DG_OK, 0, 0, 0, "");
     return DG_OK;
 }
 

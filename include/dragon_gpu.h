@@ -143,6 +143,30 @@ int dg_apply_boundary_element(dg_grid* g, const dg_bc* bc);
 int dg_cfl_dt(dg_grid* g, double* dt, dg_status* st);
 /* GridND::unsplit_step(dt) (Unsplit.cpp:28-206, Splitting.cpp:100-115): no commit on failure. */
 int dg_step(dg_grid* g, double dt, dg_status* st);
+/* dg_step_noalloc: no counterpart in the reference.  For callers that cannot
+ * allocate device memory (tight stepping loops, harnesses that forbid
+ * hipMalloc): the step of dg_step on caller-owned device buffers, with every
+ * scratch array (fluxes, half-step faces, EMFs, the failure word) in one
+ * static device array of dg_noalloc_scratch_limit() bytes (64 MiB) per GPU,
+ * held until the process ends (enough for 46^3 3D MHD or 446^2 2D MHD).  It
+ * calls no hipMalloc / hipFree.
+ *   w_in / w_out: SoA cell state, 8 fields of dg_cell_count doubles for the
+ *   grid desc describes (ghosts included, as dg_download_device); B_in /
+ *   B_out: 3 fields of dg_face_count doubles (2D/3D MHD only, else NULL).
+ *   The ghosts of w_in / B_in must be filled; w_out / B_out receive the whole
+ *   next state and must not alias the inputs.  stream: hipStream_t or NULL.
+ * Grids whose scratch (dg_noalloc_scratch_bytes) exceeds the limit are refused
+ * with DG_ERR_ARG before any device call.  Returns after the stream has
+ * synchronised; on a failed step the outputs are undefined and st holds the
+ * reference's message, as for dg_step.  All calls on one GPU share its array,
+ * which keeps the last call's contents: calls that overlap on one GPU (two
+ * streams or threads) race.  dg_noalloc_last_error: the message of this
+ * thread's last refused call. */
+int dg_step_noalloc(const dg_desc* desc, const double* w_in, const double* B_in, double* w_out, double* B_out, double dt,
+                    void* stream, dg_status* st);
+long dg_noalloc_scratch_bytes(const dg_desc* desc);
+long dg_noalloc_scratch_limit(void);
+const char* dg_noalloc_last_error(void);
 /* Grid::advance_unsplit(dt, check_cfl) (CFL.cpp:213-238); max_steps < 0: no limit. */
 int dg_advance(dg_grid* g, double dt, int check_cfl, long max_steps, dg_stats* stats, dg_status* st);
 /* Grid2D/3D::split_step(dt) (Hydro/Godunov/Splitting.cpp:118-216): Strang-split
