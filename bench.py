@@ -326,7 +326,8 @@ def main():
         inst = valu[best]["valu_insts_per_step"]
         roof["valu_issue"] = {"frac": round(inst * 4 / (1024 * 2.4e9 * r["kernel_ms_per_step"] * 1e-3), 4),
                               "valu_insts_per_step": inst, "cycles_per_inst": 4, "simds": 1024, "clock_GHz": 2.4,
-                              "source": "SQ_INSTS_VALU x SQ_WAVES per kernel class, profiles/valu_c4_512.json"}
+                              "source": "SQ_INSTS_VALU summed over the step's kernels per step, from a counter run of this command "
+                                        "(profiles/valu_c4_512.json); the fraction is a lower bound when the clock runs below 2.4 GHz"}
     cpu = None if a.no_cpu else cpu_baseline(a.cpu_n)
     line = {
         "metric": "zone-updates/s, 3D MHD HLLD+CT 512^3, mixed-FP32 vs FP64 GPU; % HBM roofline",

@@ -2,7 +2,7 @@
 
     python -m dragon_amd.profiling stats  KT_DIR OUT.md --command "..."
     python -m dragon_amd.profiling traffic FETCH_DIR WRITE_DIR OUT.json --steps 3 --forms fp64,mixed
-    python -m dragon_amd.profiling valu PMC_DIR OUT.md --command "..."
+    python -m dragon_amd.profiling valu PMC_DIR OUT.md --command "..." [--json OUT.json --steps 3]
 
 ``stats`` turns a ``rocprofv3 --kernel-trace --stats --output-format csv``
 directory into a Markdown table (per kernel: calls, average and total device
@@ -159,7 +159,7 @@ def traffic(fetch_dir, write_dir, steps, forms, commands=()):
 VALU_CYCLES, CLOCK_HZ, SIMDS = 4, 2.4e9, 1024
 
 
-def valu_table(pmc_dir: str, command: str) -> str:
+def _valu_agg(pmc_dir: str):
     disp = {}
     for f in _find(pmc_dir, "counter_collection.csv"):
         for r in csv.DictReader(open(f)):
@@ -180,6 +180,24 @@ def valu_table(pmc_dir: str, command: str) -> str:
         for k, v in d.items():
             if k.startswith("SQ_"):
                 a[k] += v
+    return agg
+
+
+def valu_per_step(pmc_dir: str, steps: int, command: str = "") -> dict:
+    """Wave-level VALU instructions per step of each form (the step's kernel
+    classes summed; the counted run advanced each form `steps` steps)."""
+    out = {}
+    for (form, cls), a in _valu_agg(pmc_dir).items():
+        f = out.setdefault(form, {"valu_insts_per_step": 0.0, "kernel_ns_per_step": 0.0, "steps": steps, "command": command})
+        f["valu_insts_per_step"] += a.get("SQ_INSTS_VALU", 0.0) / steps
+        f["kernel_ns_per_step"] += a["ns"] / steps
+    for f in out.values():
+        f["valu_issue_share_counter_run"] = round(f["valu_insts_per_step"] * VALU_CYCLES / (f["kernel_ns_per_step"] * 1e-9 * CLOCK_HZ * SIMDS), 4)
+    return out
+
+
+def valu_table(pmc_dir: str, command: str) -> str:
+    agg = _valu_agg(pmc_dir)
     lines = [f"Command: `{command}`", "",
              "| form | kernel class | launches | waves/launch | VALU/wave | VMEM rd/wave | SALU/wave | ms/launch | VALU issue share | wave cycles waiting |",
              "|---|---|---|---|---|---|---|---|---|---|"]
@@ -213,10 +231,17 @@ def main(argv=None):
     a3.add_argument("pmc_dir")
     a3.add_argument("out")
     a3.add_argument("--command", default="")
+    a3.add_argument("--json", help="also write VALU instructions per step per form (bench.py reads it)")
+    a3.add_argument("--steps", type=int, default=0, help="steps per form in the counted run (warmup + timed), for --json")
     a = ap.parse_args(argv)
     if a.cmd == "valu":
         with open(a.out, "w") as f:
             f.write(valu_table(a.pmc_dir, a.command))
+        if a.json:
+            if a.steps <= 0:
+                ap.error("--json needs --steps")
+            with open(a.json, "w") as f:
+                json.dump(valu_per_step(a.pmc_dir, a.steps, a.command), f, indent=1)
     elif a.cmd == "stats":
         text = stats_table(a.kt_dir, a.command)
         with open(a.out, "w") as f:

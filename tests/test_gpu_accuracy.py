@@ -3,10 +3,10 @@ run on the GPU's FP64 path (bit-identical to the oracle on the parity
 cases).  The runs follow main.cpp:133-150 frame by frame (advance(frame)
 while time < final_time) with each example's Config.h switches:
 
-- Examples/Diagonal/Outputs/Validation_data.md: L1(rho) of the diagonal
-  sine advection (hydro, HLLC, CTU, periodic) after t = 1;
-- Examples/LoopAdvection/Outputs/Validation_data.md: L1 of |B|^2 error / B0^2
-  of the field loop (MHD, HLLD + CT + CTU) after t = 4.
+- Examples/Diagonal/Outputs/Validation_data.md: L1, L2 and Linf of rho of
+  the diagonal sine advection (hydro, HLLC, CTU, periodic) after t = 1;
+- Examples/LoopAdvection/Outputs/Validation_data.md: L1, L2 and Linf of the
+  |B|^2 error / B0^2 of the field loop (MHD, HLLD + CT + CTU) after t = 4.
 
 The tables print six significant digits, so agreement is asked to 1e-5
 relative."""
@@ -34,55 +34,89 @@ def run_frames(pb):
 
 
 @pytest.mark.parametrize("n,limiter,published", [
-    (128, 1, 0.000224884),   # HLLC + MC
-    (256, 0, 0.000286346),   # HLLC + minmod
-    (128, 3, 0.000693177),   # HLLC + superbee
+    (128, 1, (0.000224884, 0.000310695, 0.000942581)),   # HLLC + MC
+    (256, 0, (0.000286346, 0.000449423, 0.00151213)),    # HLLC + minmod
+    (128, 3, (0.000693177, 0.00090973, 0.00278401)),     # HLLC + superbee
 ])
-def test_diagonal_2d_published_l1(n, limiter, published):
+def test_diagonal_2d_published(n, limiter, published):
     pb = P.diagonal_2d(n, limiter)
     gg, _, t, steps = run_frames(pb)
-    l1, l2, linf = P.diagonal_error(gg.get_w(), gg.g, pb[0], t)
-    print(f"diagonal 2D n={n} limiter={limiter}: {steps} steps, L1 {l1:.6g} (published {published})", flush=True)
-    assert abs(l1 - published) <= REL * published
+    got = P.diagonal_error(gg.get_w(), gg.g, pb[0], t)
+    print(f"diagonal 2D n={n} limiter={limiter}: {steps} steps, L1 L2 Linf {got} (published {published})", flush=True)
+    for v, p in zip(got, published):
+        assert abs(v - p) <= REL * p
 
 
-def test_diagonal_3d_published_l1():
-    """3D reproduces the published L1 to four digits only: 0.00116806 here
-    against 0.00116811 published (4.5e-5 relative), while the three 2D tables
-    agree in all six.  Reading CTU.cpp's 12-solve sweep, correctState and
-    DistGrid3D's child set-up against the oracle found no difference; the
-    remaining gap is recorded, and bounded here at 1e-4."""
-    pb = P.diagonal_3d(64, 1)
+@pytest.mark.parametrize("n,published,measured,bound", [
+    (64, (0.00116811, 0.00139005, 0.00291), (0.00116806, 0.00138738, 0.00289601), (1e-4, 3e-3, 6e-3)),
+    (128, (0.000321543, 0.000424485, 0.00121935), (0.000321895, 0.000424024, 0.00121545), (2e-3, 2e-3, 5e-3)),
+])
+def test_diagonal_3d_published(n, published, measured, bound):
+    """3D reproduces the published table to three or four digits only (L1
+    4.5e-5 relative at n=64, 1.1e-3 at n=128), while the three 2D tables
+    agree in all six.  CTU.cpp's correctState and 12-solve computeCTUFlux
+    built from the reference's own source agree with the oracle bit for bit
+    (golden_ctu), and DistGrid3D's decomposition (27 children with a ghost
+    exchange before every step, Refinement/DistGrid.cpp:84-111, 162-275)
+    emulated with oracle children gives the single grid's norms to the last
+    bit, so neither explains the gap.  The norms are pinned to this path's
+    values (1e-5) and held within the stated bound of the published ones."""
+    pb = P.diagonal_3d(n, 1)
     gg, _, t, steps = run_frames(pb)
-    l1, _, _ = P.diagonal_error(gg.get_w(), gg.g, pb[0], t)
-    print(f"diagonal 3D n=64 MC: {steps} steps, L1 {l1:.6g} (published 0.00116811)", flush=True)
-    assert abs(l1 - 0.00116811) <= 1e-4 * 0.00116811
+    got = P.diagonal_error(gg.get_w(), gg.g, pb[0], t)
+    print(f"diagonal 3D n={n} MC: {steps} steps, L1 L2 Linf {got} (published {published})", flush=True)
+    for v, m, p, b in zip(got, measured, published, bound):
+        assert abs(v - m) <= REL * m
+        assert abs(v - p) <= b * p
 
 
-@pytest.mark.parametrize("n,gaussian,published,measured", [(64, False, 0.0286387, 0.0265791), (128, False, 0.0172391, 0.0152281),
-                                                         (64, True, 0.601428, None)])
-def test_loop_advection_2d_recorded_deviation(n, gaussian, published, measured):
-    """2D field loops do NOT reproduce the published table: the uniform-B
-    loop comes out 7% (n=64) and 12% (n=128) lower than published, while the
-    3D loop below agrees in all six digits and every 2D CT piece (corner EMF
-    and its upwinding, Faraday, body fields) is pinned bit-exact to the
-    reference's own code by the golden vectors.  Reading the 2D
-    ctu_sweep_MHD, computeHalfStates and Grid2D::unsplit_step against the
-    oracle found no difference.  Pinned here to the value this path gives
-    (1e-5), with the published value printed beside it."""
-    pb = P.loop_advection(n, 2, gaussian=gaussian)
+# (dimension, gaussian A_z, n) -> published L1, L2, Linf of the |B|^2 error
+# (Examples/LoopAdvection/Outputs/Validation_data.md)
+LOOP_PUBLISHED = {
+    (2, True, 64): (0.601428, 3.53229, 31.4691),
+    (3, False, 64): (0.0139403, 0.107475, 0.999567),
+    (3, False, 128): (0.00866564, 0.066301, 0.998004),
+    (3, True, 64): (0.323758, 2.65926, 32.7321),
+}
+
+
+def loop_norms(n, dim, gaussian):
+    pb = P.loop_advection(n, dim, gaussian=gaussian)
     gg, w0, t, steps = run_frames(pb)
-    l1, _, _ = P.loop_error(gg.get_w(), w0, gg.g, pb[0], pb[6]["B0"])
-    print(f"loop 2D n={n} gaussian={gaussian}: {steps} steps, L1 {l1:.6g} (published {published}), div B {gg.divb():.3g}",
-          flush=True)
+    got = P.loop_error(gg.get_w(), w0, gg.g, pb[0], pb[6]["B0"])
+    print(f"loop {dim}D n={n} gaussian={gaussian}: {steps} steps, L1 L2 Linf {got}, div B {gg.divb():.3g}", flush=True)
     assert gg.divb() < 1e-12
-    if measured is not None:
-        assert abs(l1 - measured) <= REL * measured
+    return got
 
 
-def test_loop_advection_3d_gaussian_published_l1():
-    pb = P.loop_advection(64, 3, gaussian=True)
-    gg, w0, t, steps = run_frames(pb)
-    l1, _, _ = P.loop_error(gg.get_w(), w0, gg.g, pb[0], pb[6]["B0"])
-    print(f"loop 3D gaussian n=64: {steps} steps, L1 {l1:.6g} (published 0.323758)", flush=True)
-    assert abs(l1 - 0.323758) <= REL * 0.323758
+@pytest.mark.parametrize("dim,gaussian,n", list(LOOP_PUBLISHED))
+def test_loop_advection_published(dim, gaussian, n):
+    """The field loop (MHD, HLLD + CT + CTU, periodic) after t = 4: L1, L2
+    and Linf of the |B|^2 error agree with the published table in all six
+    printed digits."""
+    got = loop_norms(n, dim, gaussian)
+    for v, p in zip(got, LOOP_PUBLISHED[(dim, gaussian, n)]):
+        assert abs(v - p) <= REL * p
+
+
+@pytest.mark.xfail(strict=True, reason="the 2D uniform-B loop gives L1 7% (n=64) and 12% (n=128) below the published "
+                   "table while its Linf agrees; the 2D Gaussian loop and both 3D loops agree in every norm")
+@pytest.mark.parametrize("n,published", [(64, (0.0286387, 0.152378, 0.996704)), (128, (0.0172391, 0.0949274, 0.997576))])
+def test_loop_advection_2d_uniform_published(n, published):
+    got = loop_norms(n, 2, False)
+    for v, p in zip(got, published):
+        assert abs(v - p) <= REL * p
+
+
+@pytest.mark.parametrize("n,measured", [(64, (0.0265791, 0.144360, 0.996704)), (128, (0.0152281, 0.0846040, 0.997576))])
+def test_loop_advection_2d_uniform_recorded(n, measured):
+    """The 2D uniform-B loop does not reproduce the published L1 and L2.
+    Every 2D CT piece (corner EMF and its upwinding, Faraday, body fields)
+    is pinned bit-exact to the reference's own code by the golden vectors,
+    the same kernels reproduce the 2D Gaussian loop and the 3D loops in all
+    three norms, and the published Linf agrees here too; the difference is
+    confined to this one initial condition.  Pinned to this path's values
+    (1e-5), with the published values in the xfail test above."""
+    got = loop_norms(n, 2, False)
+    for v, m in zip(got, measured):
+        assert abs(v - m) <= REL * m

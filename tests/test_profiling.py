@@ -65,3 +65,9 @@ def test_valu_table(tmp_path):
     assert cells[0] == "mixed" and cells[2] == "2" and cells[3] == "256000" and cells[4] == "1200"
     # 256000 * 1200 * 4 cycles / (1 ms * 2.4 GHz * 1024 SIMDs) = 0.5
     assert cells[8] == "0.50" and cells[9] == "0.25"
+    # per step: the counted run advanced the form 2 steps, one launch each
+    per = P.valu_per_step(str(tmp_path / "p"), steps=2, command="cmd")
+    assert set(per) == {"mixed"}
+    assert per["mixed"]["valu_insts_per_step"] == 256000 * 1200
+    assert per["mixed"]["kernel_ns_per_step"] == 1_000_000
+    assert per["mixed"]["valu_issue_share_counter_run"] == 0.5
