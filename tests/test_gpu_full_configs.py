@@ -82,14 +82,19 @@ def rel_l1_vars(a, b):
 
 def test_c3_orszag_tang_2048_to_t05():
     """Both forms advanced side by side in two frames, t = 0.25 and t = 0.5.
-    The mixed form meets the 1e-5 bar on L1(rho) at t = 0.25 but not at
-    t = 0.5: between the two the flow (shocks and current sheets forming)
-    amplifies any perturbation 20-150-fold, and the FP32 Riemann fluxes
-    perturb each step at the 1e-7 level.  Measured at t = 0.5: 6.3e-5 to
-    1.2e-4 at 2048^2 over builds of the mixed kernels (8.7e-5 at 1024^2; 5.3e-5
-    with correctly rounded FP32 division and square root, 9.6e-5 without FMA
-    contraction).  At t = 0.25 it is 2.0e-6 to 4.3e-6.  Recorded deviation: bounded here
-    at 2e-4, the value printed."""
+    The mixed form meets the 1e-5 bar on L1(rho) at t = 0.25 (2.0e-6 to
+    4.3e-6 over builds of the mixed kernels) but not at t = 0.5 (6.3e-5 to
+    1.2e-4; 1.2e-4 with the current build), and no form that rounds any stage
+    to FP32 can: two controls at 2048^2, t = 0.5, against the FP64 form,
+    - FP64 form from an initial state perturbed by one ulp: 4.4e-11, so the
+      flow does not amplify round-off of the FP64 path beyond the bar;
+    - every stage in FP64 and only the flux arrays stored as FP32 (a
+      build that differs from the FP64 form in that one type): 3.1e-5 (1.4e-6
+      at t = 0.25), three times the bar from storage rounding alone.
+    The 1e-5 bar at t = 0.5 therefore needs FP64 storage and FP64
+    reconstruction and Riemann solves, i.e. the FP64 form.  This test holds
+    the mixed form to 1e-5 at t = 0.25 and to 2e-4 at t = 0.5, the printed
+    value."""
     pb = P.orszag_tang_2d(2048)
     grids = [gpu_grid(pb, 0), gpu_grid(pb, 1)]
     start = [totals_2d(G, pb[0], True) for G in grids]
