@@ -201,7 +201,7 @@ def bench_batch(torch, npatch=4096, steps=10, seed=20260705, baseline_patches=25
         ms_graph = b.run(steps, graph=True, timed=True)
         b.sync()
         z = b.zones
-        out[form] = {"tiles": tiles, "autotune_class_ms": cls_ms,
+        out[form] = {"tiles": tiles, "ysweep": b.ysweep(), "autotune_class_ms": cls_ms,
                      "eager_zone_updates_per_s": z * steps / (ms_eager * 1e-3),
                      "graph_zone_updates_per_s": z * steps / (ms_graph * 1e-3),
                      "ms_per_step_eager": ms_eager / steps, "ms_per_step_graph": ms_graph / steps}
