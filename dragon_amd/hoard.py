@@ -260,7 +260,9 @@ class FrameWriter:
     """Frames written while the grid keeps stepping: ``submit`` starts the
     device snapshot into one of two pinned buffers and returns; a background
     thread waits for the copy and writes the file (the HDF5 call releases the
-    GIL).  ``close`` (or leaving a ``with`` block) waits for the last file."""
+    GIL; the image's HDF5 library is built thread-safe, so the two writer
+    threads serialise inside it).  ``close`` (or leaving a ``with`` block)
+    waits for the last file."""
 
     def __init__(self, grid, **options):
         self.grid = grid

@@ -14,6 +14,7 @@
 // The state lives on the GPU.  Host access goes through a mirror: pull()
 // downloads it, push() uploads edits (the reference reads grid[i,j] directly).
 #pragma once
+#include <exception>
 #include <memory>
 #include <stdexcept>
 #include <string>
@@ -191,6 +192,8 @@ public:
         h_.reset(h);
         g_ = dg_ghosts(h);
         for (int q = 0; q < 3; q++) n_[q] = q < dim ? n[q] : 1;
+        for (int q = 0; q < 3; q++) d_[q] = q < dim ? d[q] : 1.0;
+        cfg_ = c;
         has_B_ = c.mhd && dim >= 2;
         w_.assign(dg_cell_count(h) * 8, 0.0);
         if (has_B_) B_.assign(dg_face_count(h) * 3, 0.0);
@@ -203,6 +206,17 @@ public:
     int getSizeX() const { return n_[0]; }
     int getSizeY() const { return n_[1]; }
     int getSizeZ() const { return n_[2]; }
+    int dim() const { return dim_; }
+    // cell spacing per dimension (grid.dx / dy / dz)
+    double spacing(int q) const { return d_[q]; }
+    // grid.dx = ... (as loadFromFile sets them)
+    void set_spacing(double dx, double dy, double dz) {
+        check(dg_set_spacing(h_.get(), dx, dy, dz));
+        d_[0] = dx;
+        if (dim_ >= 2) d_[1] = dy;
+        if (dim_ >= 3) d_[2] = dz;
+    }
+    const Config& config() const { return cfg_; }
     dg_grid* handle() const { return h_.get(); }
 
     // host mirror of grid[i,j,k] and _B()[i,j,k] (ghost-offset indices)
@@ -219,10 +233,15 @@ public:
     // uncovered faces, then the elements in order
     void apply_boundary() {
         push_bc();
-        if (!boundary.has_host()) {
-            check(dg_apply_boundary(h_.get()));
-            return;
-        }
+        check(dg_apply_boundary(h_.get()));  // with host elements: the callback below runs apply_list()
+    }
+
+protected:
+    // The list with host elements, in the reference's order; every boundary
+    // pass of the library calls it through the boundary callback (the advance
+    // loops, dg_init_B, the pass before each directional sweep of the split
+    // step, which then acts on that sweep's input state).
+    void apply_list() {
         int covered = 0;
         for (const auto& b : boundary.items) covered |= b.faces;
         bool fresh = false, dirty = false;  // host mirror current / edited since the download
@@ -243,6 +262,21 @@ public:
         }
         if (dirty) push();
     }
+    static int bc_callback(void* user) {  // no exception may cross the C library
+        GpuGrid* g = static_cast<GpuGrid*>(user);
+        try {
+            g->apply_list();
+            return DG_OK;
+        } catch (const GpuError& e) {
+            g->cb_error_ = std::current_exception();
+            return e.code ? e.code : DG_ERR_ARG;
+        } catch (...) {
+            g->cb_error_ = std::current_exception();
+            return DG_ERR_ARG;
+        }
+    }
+
+public:
     double cfl_time() {
         push_bc();
         double dt = 0;
@@ -252,8 +286,7 @@ public:
     }
     void initialize_B_fields() {
         push_bc();
-        if (boundary.has_host()) apply_boundary();  // the device list is Ignore: dg_init_B's own pass does nothing
-        check(dg_init_B(h_.get()));
+        check(dg_init_B(h_.get()));  // its boundary pass runs the list (host elements through the callback)
     }
     // Grid::unsplit_step(dt): throws on failure, commits nothing
     virtual void unsplit_step(double dt) {
@@ -277,9 +310,6 @@ public:
 protected:
     long advance_loop(double dt, bool check_cfl, bool split) {
         push_bc();
-        if (split && boundary.has_host())
-            throw GpuError(DG_ERR_UNSUPPORTED, "the split step applies its boundaries on the device between sweeps; "
-                                               "host-evaluated GhostFill elements need the unsplit step");
         long steps = 0;
         while (dt > tol_) {
             apply_boundary();
@@ -311,16 +341,25 @@ protected:
         return ((long)(i + g_) * ey + (dim_ >= 2 ? j + g_ : 0)) * ez + (dim_ >= 3 ? k + g_ : 0);
     }
     void push_bc() {
-        if (boundary.has_host()) {  // the library's own boundary passes must do nothing; apply_boundary() sequences
+        if (boundary.has_host()) {  // the library's boundary passes call back into apply_list()
             dg_bc ign{};
             ign.kind = DG_BC_IGNORE;
             ign.faces = 63;
             check(dg_set_boundary(h_.get(), &ign, 1));
+            check(dg_set_boundary_callback(h_.get(), &GpuGrid::bc_callback, this));
             return;
         }
+        check(dg_set_boundary_callback(h_.get(), nullptr, nullptr));
         check(dg_set_boundary(h_.get(), boundary.items.data(), (int)boundary.items.size()));
     }
-    void check(int rc, const dg_status* st = nullptr) const {
+    // A failed call raises the exception a host element threw during it, if
+    // one did, else GpuError with the library's message.
+    void check(int rc, const dg_status* st = nullptr) {
+        if (cb_error_) {
+            std::exception_ptr e = cb_error_;
+            cb_error_ = nullptr;
+            std::rethrow_exception(e);
+        }
         if (!rc) return;
         std::string m = st && st->msg[0] ? std::string(st->msg) : std::string(dg_last_error(h_.get()));
         throw GpuError(rc, m);
@@ -330,6 +369,9 @@ protected:
     };
     std::unique_ptr<dg_grid, Del> h_;
     int dim_ = 1, g_ = 0, n_[3] = {1, 1, 1};
+    double d_[3] = {1, 1, 1};
+    Config cfg_;
+    std::exception_ptr cb_error_;
     bool has_B_ = false;
     double tol_ = 1e-14;
     bool unsplit_ = true;

@@ -6,11 +6,19 @@
 //       checks that need no GPU: bad descriptions throw GpuError(DG_ERR_ARG).
 //   grid_driver hostbc
 //       a host-evaluated boundary element against the built-in one it restates,
-//       the split step's refusal of host elements, advance() in split mode.
+//       under the unsplit and the split step; a host element's exception
+//       reaching the caller; advance() in split mode.
+//   grid_driver hoard DIR
+//       frames through the C++ DRAGONHOARD mirror (dragon_hoard.hpp): a 3D MHD
+//       run written at cycle 7 (cell B in double), restarted into a grid of
+//       other spacing, both advanced alike and compared bit for bit; DIR gets
+//       the frame and state.bin, the state it holds (cells, ghosts included,
+//       8 doubles each, grid order).
 //   grid_driver run DIM NX NY NZ DX DY DZ MHD SOLVER LIMITER CT_ENERGY GAMMA BC T_END IN OUT
 //       BC: periodic | outflow.  IN holds the cells (ghosts included, 8
 //       doubles each, grid order) and, for MHD in 2D/3D, the faces (3 doubles
 //       each); OUT gets the same layout after advance(T_END).
+#include <cmath>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
@@ -18,6 +26,7 @@
 #include <vector>
 
 #include "dragon_gpu.hpp"
+#include "dragon_hoard.hpp"
 
 using namespace dragon_gpu;
 
@@ -69,7 +78,7 @@ struct HostFixed : HostGhostFill {
 };
 
 // A host element in a boundary list gives the bits of the built-in element it
-// restates; the split step refuses host elements; Config.dimension_unsplit = 0
+// restates, under the unsplit and the split step; Config.dimension_unsplit = 0
 // makes advance() the split advance.
 static int hostbc_checks() {
     Config c;
@@ -105,12 +114,39 @@ static int hostbc_checks() {
             if (std::memcmp(&a.cell(i, j), &b.cell(i, j), sizeof(State))) failures++;
     std::printf("host element vs Fixed: %ld / %ld steps, %d cells differ\n", sa, sb, failures);
     if (sa != sb) failures++;
-    try {
-        b.advance_split(0.01);
-        std::printf("FAIL split step accepted a host element\n");
-        failures++;
-    } catch (const GpuError& e) {
-        if (e.code != DG_ERR_UNSUPPORTED) failures++;
+    // the split step applies the list before every directional sweep (Splitting.cpp:219-299): the host
+    // element runs there too, through the boundary callback
+    GpuGrid2D a2(n, n, 1.0 / n, 1.0 / n, 2, c), b2(n, n, 1.0 / n, 1.0 / n, 2, c);
+    fill(a2);
+    fill(b2);
+    a2.boundary = Fixed(inflow, X_negative) + Periodic(Y) + Outflow(X_positive);
+    b2.boundary = BoundaryList(std::make_shared<HostFixed>(inflow, X_negative)) + Periodic(Y) + Outflow(X_positive);
+    const long ta = a2.advance_split(0.02), tb = b2.advance_split(0.02);
+    a2.pull();
+    b2.pull();
+    int dsplit_host = 0;
+    for (int i = -G; i < n + G; i++)
+        for (int j = -G; j < n + G; j++)
+            if (std::memcmp(&a2.cell(i, j), &b2.cell(i, j), sizeof(State))) dsplit_host++;
+    std::printf("split step, host element vs Fixed: %ld / %ld steps, %d cells differ\n", ta, tb, dsplit_host);
+    failures += dsplit_host + (ta != tb);
+    // an exception thrown by a host element stops the call and reaches the caller
+    struct Broken : HostGhostFill {
+        Broken() : HostGhostFill(X_negative, true) {}
+        void apply(GpuGrid&) override { throw std::runtime_error("user boundary failed"); }
+    };
+    b2.boundary = BoundaryList(Periodic()) + std::make_shared<Broken>();
+    for (int split = 0; split < 2; split++) {
+        try {
+            split ? b2.advance_split(1e-3) : b2.advance(1e-3);
+            std::printf("FAIL host element exception swallowed\n");
+            failures++;
+        } catch (const GpuError&) {
+            std::printf("FAIL host element exception turned into GpuError\n");
+            failures++;
+        } catch (const std::runtime_error& e) {
+            if (std::string(e.what()) != "user boundary failed") failures++;
+        }
     }
     Config cs = c;
     cs.dimension_unsplit = 0;
@@ -132,9 +168,88 @@ static int hostbc_checks() {
     return failures ? 1 : 0;
 }
 
+// A restart from a frame continues the run bit for bit (main.cpp:71-97).
+static int hoard_checks(const std::string& dir) {
+    Config c;
+    c.limiter = DG_VANLEER;
+    const int nx = 12, ny = 10, nz = 14;
+    const double pi2 = 6.283185307179586;
+    auto make = [&](double dx) { return GpuGrid3D(nx, ny, nz, dx, dx, dx, 3, c); };
+    GpuGrid3D a = make(1.0 / 12);
+    const int G = a.getGhosts();
+    for (int i = 0; i < nx; i++)
+        for (int j = 0; j < ny; j++)
+            for (int k = 0; k < nz; k++) {
+                State w;
+                w.rho = 1.0 + 0.2 * std::sin(pi2 * i / nx) * std::cos(pi2 * k / nz);
+                w.p = 1.0;
+                w.vx = 0.3 * std::sin(pi2 * j / ny);
+                w.vy = 0.2 * std::cos(pi2 * k / nz);
+                w.vz = 0.1 * std::sin(pi2 * i / nx);
+                a.cell(i, j, k) = w;
+            }
+    for (int i = -G; i < nx + 1 + G; i++)  // each component independent of its own coordinate: div B = 0
+        for (int j = -G; j < ny + 1 + G; j++)
+            for (int k = -G; k < nz + 1 + G; k++) {
+                Vec3& b = a.face(i, j, k);
+                b.x = 0.4 + 0.1 * std::sin(pi2 * (j + 0.5) / ny);
+                b.y = 0.1 * std::cos(pi2 * (k + 0.5) / nz);
+                b.z = 0.1 * std::sin(pi2 * (i + 0.5) / nx);
+            }
+    a.push();
+    a.boundary = Periodic();
+    a.initialize_B_fields();
+    a.advance(0.02);
+    hoard::Options o;
+    o.output_dir = dir;
+    o.redundant = DGH_WRITE_DOUBLE;  // cell B in double: the restart is exact
+    hoard::writeToFile(a, 0.02, 7, "frame_" + hoard::cycle_string(7), o);
+    a.pull();
+    {
+        FILE* f = std::fopen((dir + "/state.bin").c_str(), "wb");
+        if (!f) return 1;
+        const long nc = (long)(nx + 2 * G) * (ny + 2 * G) * (nz + 2 * G);
+        std::fwrite(&a.cell(-G, -G, -G), sizeof(double), 8 * nc, f);
+        std::fclose(f);
+    }
+    GpuGrid3D b = make(0.5);  // spacing replaced by the file's
+    b.boundary = Periodic();
+    double t = -1;
+    int cycle = -1;
+    hoard::loadFromFile(b, t, cycle, "frame_" + hoard::cycle_string(7), o);
+    int failures = 0;
+    if (t != 0.02 || cycle != 7 || b.spacing(0) != a.spacing(0) || b.spacing(2) != a.spacing(2)) {
+        std::printf("FAIL restart metadata: t %g cycle %d dx %g\n", t, cycle, b.spacing(0));
+        failures++;
+    }
+    const long sa = a.advance(0.02), sb = b.advance(0.02);
+    a.pull();
+    b.pull();
+    int dc = 0, df = 0;
+    for (int i = 0; i < nx; i++)
+        for (int j = 0; j < ny; j++)
+            for (int k = 0; k < nz; k++)
+                if (std::memcmp(&a.cell(i, j, k), &b.cell(i, j, k), sizeof(State))) dc++;
+    for (int i = 0; i <= nx; i++)
+        for (int j = 0; j <= ny; j++)
+            for (int k = 0; k <= nz; k++)
+                if (std::memcmp(&a.face(i, j, k), &b.face(i, j, k), sizeof(Vec3))) df++;
+    std::printf("restart from frame_00007: %ld / %ld steps after it, %d cells and %d faces differ\n", sa, sb, dc, df);
+    failures += dc + df + (sa != sb);
+    try {
+        hoard::loadFromFile(b, t, cycle, "no_such_frame", o);
+        std::printf("FAIL missing file accepted\n");
+        failures++;
+    } catch (const hoard::HoardError&) {
+    }
+    std::printf(failures ? "hoard checks failed\n" : "hoard checks passed\n");
+    return failures ? 1 : 0;
+}
+
 int main(int argc, char** argv) {
     if (argc >= 2 && !std::strcmp(argv[1], "abi")) return abi_checks();
     if (argc >= 2 && !std::strcmp(argv[1], "hostbc")) return hostbc_checks();
+    if (argc >= 3 && !std::strcmp(argv[1], "hoard")) return hoard_checks(argv[2]);
     if (argc != 18 || std::strcmp(argv[1], "run")) {
         std::fprintf(stderr, "usage: grid_driver abi | run DIM NX NY NZ DX DY DZ MHD SOLVER LIMITER CT_ENERGY GAMMA BC T_END IN OUT\n");
         return 2;

@@ -1,7 +1,8 @@
-"""The C++ mirror of the Grid / Boundary API (include/dragon_gpu.hpp), driven
-by tests/cpp/grid_driver (built by __graft_entry__.build()): argument
-checks on the CPU, and on the GPU a Sod tube and an Orszag-Tang run through
-the C++ classes, bit-identical to the oracle."""
+"""The C++ mirror of the Grid / Boundary API (include/dragon_gpu.hpp) and of
+DRAGONHOARD (include/dragon_hoard.hpp), driven by tests/cpp/grid_driver (built
+by __graft_entry__.build()): argument checks on the CPU, and on the GPU a Sod
+tube and an Orszag-Tang run through the C++ classes, bit-identical to the
+oracle, host boundary elements and frames / restarts."""
 import os
 import subprocess
 
@@ -71,10 +72,32 @@ def test_cpp_api_orszag_tang_matches_oracle(tmp_path):
 @pytest.mark.gpu
 def test_cpp_api_host_boundary_element_and_split_mode():
     """HostGhostFill in a BoundaryList (the C++ counterpart of a user GhostFill
-    subclass) gives the same bits as the built-in Fixed it restates; the split
-    step refuses it; Config.dimension_unsplit = 0 makes advance() split."""
+    subclass) gives the same bits as the built-in Fixed it restates, under the
+    unsplit and the split step; its exceptions reach the caller;
+    Config.dimension_unsplit = 0 makes advance() split."""
     need_driver()
     r = subprocess.run([DRIVER, "hostbc"], capture_output=True, text=True, timeout=300)
     print(r.stdout)
     assert r.returncode == 0, r.stdout + r.stderr
     assert "hostbc checks passed" in r.stdout
+
+
+@pytest.mark.gpu
+def test_cpp_api_frames_and_restart(tmp_path):
+    """The C++ DRAGONHOARD mirror (include/dragon_hoard.hpp): a restart from a
+    frame written by writeToFile continues the run bit for bit (the driver
+    checks that), and the frame holds the state the grid had, in the
+    reference's file order, as the Python reader reads it."""
+    need_driver()
+    from dragon_amd import hoard as H
+    r = subprocess.run([DRIVER, "hoard", str(tmp_path)], capture_output=True, text=True, timeout=300)
+    print(r.stdout, r.stderr)
+    assert r.returncode == 0 and "hoard checks passed" in r.stdout
+    meta = H.read_meta(str(tmp_path / "frame_00007.h5"))
+    assert (meta.dim, meta.mhd, meta.nx, meta.ny, meta.nz, meta.ng, meta.cycle) == (3, 1, 12, 10, 14, 0, 7)
+    assert meta.time == 0.02 and meta.dx == 1.0 / 12
+    _, cells, _ = H.read_arrays(str(tmp_path / "frame_00007.h5"), meta)
+    g = 3
+    state = np.fromfile(str(tmp_path / "state.bin")).reshape(12 + 2 * g, 10 + 2 * g, 14 + 2 * g, 8)[g:-g, g:-g, g:-g]
+    want = np.ascontiguousarray(np.transpose(state, (2, 1, 0, 3))).reshape(-1, 8).T  # x fastest
+    assert np.array_equal(cells, want)
