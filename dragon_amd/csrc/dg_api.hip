@@ -164,6 +164,16 @@ static StepCtx make_ctx(dg_grid* g, double dt) {
     return c;
 }
 
+// Device code indexes each array in 32 bits: the face index space (the
+// largest array) must hold fewer than DG_MAX_ELEMS elements, which on one
+// GPU's memory only the largest 1D grids exceed.
+static bool fits_index_space(const dg_desc& d, int ghosts) {
+    const long ex = d.nx + 1 + 2L * ghosts, ey = d.dim >= 2 ? d.ny + 1 + 2L * ghosts : 1, ez = d.dim >= 3 ? d.nz + 1 + 2L * ghosts : 1;
+    if (ex >= DG_MAX_ELEMS || ey >= DG_MAX_ELEMS || ez >= DG_MAX_ELEMS) return false;
+    const long exy = ex * ey;
+    return exy < DG_MAX_ELEMS && exy * ez < DG_MAX_ELEMS;
+}
+
 extern "C" {
 
 int dg_create(const dg_desc* desc, dg_grid** out) {
@@ -173,6 +183,7 @@ int dg_create(const dg_desc* desc, dg_grid** out) {
     if (d.dim < 1 || d.dim > 3 || d.nx < 1 || (d.dim >= 2 && d.ny < 1) || (d.dim >= 3 && d.nz < 1)) return DG_ERR_ARG;
     if (!(d.dx > 0) || (d.dim >= 2 && !(d.dy > 0)) || (d.dim >= 3 && !(d.dz > 0))) return DG_ERR_ARG;
     if (d.ctu && !d.muscl) return DG_ERR_ARG;  // CTU.cpp:26-30
+    if (!fits_index_space(d, std::max(d.ghosts, 3))) return DG_ERR_ARG;
     g = new dg_grid();
     g->d = d;
     if (g->d.timestep_tol <= 0) g->d.timestep_tol = 1e-14;
@@ -886,6 +897,7 @@ extern "C" int dg_step_noalloc(const dg_desc* desc, const double* w_in, const do
         return noalloc_fail(DG_ERR_ARG, "dg_step_noalloc: bad sizes");
     if (!(d.dx > 0) || (d.dim >= 2 && !(d.dy > 0)) || (d.dim >= 3 && !(d.dz > 0)) || (d.ctu && !d.muscl))
         return noalloc_fail(DG_ERR_ARG, "dg_step_noalloc: bad spacing or switches");
+    if (!fits_index_space(d, std::max(d.ghosts, 3))) return noalloc_fail(DG_ERR_ARG, "dg_step_noalloc: grid too large");
     const bool faces = d.mhd && d.dim >= 2;
     if (faces && (!B_in || !B_out)) return noalloc_fail(DG_ERR_ARG, "dg_step_noalloc: face B buffers required for 2D/3D MHD");
     if (w_in == w_out || (faces && B_in == B_out)) return noalloc_fail(DG_ERR_ARG, "dg_step_noalloc: outputs alias the inputs");
@@ -1090,8 +1102,10 @@ int dg_batch_create(const dg_desc* physics, int npatch, const int* nx, const int
                     int bc_kind, dg_batch** out) {
     if (!physics || !out || npatch < 1 || !nx || !ny || !dx || !dy) return DG_ERR_ARG;
     if (!physics->mhd || !physics->ctu || !physics->muscl || (bc_kind != DG_BC_OUTFLOW && bc_kind != DG_BC_PERIODIC)) return DG_ERR_ARG;
-    for (int p = 0; p < npatch; p++)
+    for (int p = 0; p < npatch; p++) {
         if (nx[p] < 4 || ny[p] < 4 || !(dx[p] > 0) || !(dy[p] > 0)) return DG_ERR_ARG;
+        if ((nx[p] + 7L) * (ny[p] + 7L) >= DG_MAX_ELEMS) return DG_ERR_ARG;  // 32-bit indexing within a patch (3 ghosts)
+    }
     dg_batch* b = new dg_batch();
     b->d = *physics;
     b->d.dim = 2;

@@ -14,6 +14,11 @@ namespace dg {
 
 enum { F_RHO = 0, F_P = 1, F_VX = 2, F_VY = 3, F_VZ = 4, F_BX = 5, F_BY = 6, F_BZ = 7 };
 
+// Every array of a grid holds fewer than DG_MAX_ELEMS elements (dg_create
+// refuses larger grids), so device code indexes in 32 bits: element offsets
+// below 2^31 and byte offsets below 2^32 from a field's base pointer.
+constexpr long DG_MAX_ELEMS = 1L << 29;
+
 // Index space of one ExtendedArray (cells: n, faces/edges: n+1).
 struct Ext {
     int n[3];
@@ -21,7 +26,9 @@ struct Ext {
     long s[3];    // strides of i, j, k (0 for unused dims)
     long base;    // offset of index (0,0,0)
     long size;    // total elements incl. ghosts
-    __host__ __device__ long at(int i, int j, int k) const { return base + i * s[0] + j * s[1] + k * s[2]; }
+    __host__ __device__ long at(int i, int j, int k) const {
+        return (long)((int)base + i * (int)s[0] + j * (int)s[1] + k * (int)s[2]);
+    }
 };
 
 __host__ __device__ inline Ext make_ext(int dim, int nx, int ny, int nz, int g) {
@@ -44,10 +51,11 @@ struct Box {
         return (long)(hi[0] - lo[0]) * (hi[1] - lo[1]) * (hi[2] - lo[2]);
     }
     __device__ bool point(long t, int& i, int& j, int& k) const {
-        const long nk = hi[2] - lo[2], nj = hi[1] - lo[1];
         if (t >= count()) return false;
-        k = lo[2] + (int)(t % nk);
-        long r = t / nk;
+        const unsigned nk = hi[2] - lo[2], nj = hi[1] - lo[1];
+        unsigned r = (unsigned)t;
+        k = lo[2] + (int)(r % nk);
+        r /= nk;
         j = lo[1] + (int)(r % nj);
         i = lo[0] + (int)(r / nj);
         return true;
@@ -66,7 +74,9 @@ template <class T>
 struct SoaT {
     T* p;
     long stride;
-    __device__ T& operator()(int f, long m) const { return p[f * stride + m]; }
+    __device__ T& operator()(int f, long m) const {  // the field's base, then a 32-bit byte offset
+        return *reinterpret_cast<T*>(reinterpret_cast<char*>(p + f * stride) + (unsigned)((unsigned)m * (unsigned)sizeof(T)));
+    }
 };
 using Soa = SoaT<double>;
 

@@ -1006,9 +1006,9 @@ __device__ __forceinline__ void faraday_body(const ET* Ex, const ET* Ey, const E
     if (m >= fe.size) return;
     double bx = Bix[m], by = Biy[m], bz = Biz[m];
     // recover (i,j,k) of m
-    long r = m;
+    unsigned r = (unsigned)m;
     int i, j = 0, k = 0;
-    const int Ny = D >= 2 ? fe.n[1] + 2 * fe.g : 1, Nz = D >= 3 ? fe.n[2] + 2 * fe.g : 1;
+    const unsigned Ny = D >= 2 ? fe.n[1] + 2 * fe.g : 1, Nz = D >= 3 ? fe.n[2] + 2 * fe.g : 1;
     if (D == 3) { k = (int)(r % Nz) - fe.g; r /= Nz; }
     if (D >= 2) { j = (int)(r % Ny) - fe.g; r /= Ny; }
     i = (int)r - fe.g;
@@ -1177,9 +1177,9 @@ __device__ __forceinline__ double apply_body(const Soa& W, const Soa& Wn, const 
                                              double cdx = 1, double cdy = 1, double cdz = 1) {
     if (m >= ce.size) return 0.0;
     const int nf = MHD ? 8 : 5;
-    long r = m;
+    unsigned r = (unsigned)m;
     int i, j = 0, k = 0;
-    const int Ny = D >= 2 ? ce.n[1] + 2 * ce.g : 1, Nz = D >= 3 ? ce.n[2] + 2 * ce.g : 1;
+    const unsigned Ny = D >= 2 ? ce.n[1] + 2 * ce.g : 1, Nz = D >= 3 ? ce.n[2] + 2 * ce.g : 1;
     if (D == 3) { k = (int)(r % Nz) - ce.g; r /= Nz; }
     if (D >= 2) { j = (int)(r % Ny) - ce.g; r /= Ny; }
     i = (int)r - ce.g;
@@ -1912,9 +1912,9 @@ KB __global__ void k_apply_axis(Soa W, Soa Wn, FSoa F, Ext ce, double dt_d, Phys
     const long m = tid();
     if (m >= ce.size) return;
     const int g = ce.g, n = ce.n[AX];
-    long r = m;
+    unsigned r = (unsigned)m;
     int idx[3] = {0, 0, 0};
-    const int Ny = D >= 2 ? ce.n[1] + 2 * g : 1, Nz = D >= 3 ? ce.n[2] + 2 * g : 1;
+    const unsigned Ny = D >= 2 ? ce.n[1] + 2 * g : 1, Nz = D >= 3 ? ce.n[2] + 2 * g : 1;
     if (D == 3) { idx[2] = (int)(r % Nz) - g; r /= Nz; }
     if (D >= 2) { idx[1] = (int)(r % Ny) - g; r /= Ny; }
     idx[0] = (int)r - g;

@@ -68,6 +68,15 @@ def test_hydro_3d_ctu12_100_steps():
     assert run_pair(P.diagonal_3d(24), steps=100, nf=5)["bit_equal"]
 
 
+@pytest.mark.parametrize("dim,n,steps", [(2, 64, 150), (3, 24, 100)])
+def test_uniform_field_loop_100_steps(dim, n, steps):
+    """The uniform-B field loop (B identically zero outside the loop and the
+    kink at its edge: the zero-field HLLD and the zero-slope limiter paths)
+    against the oracle, bit for bit; the 2D case is the one whose published
+    table this path does not reproduce."""
+    assert run_pair(P.loop_advection(n, dim, gaussian=False), steps=steps)["bit_equal"]
+
+
 def test_fp64_parity_100_steps_2d_mhd():
     """2D MHD at a size that is not a multiple of the tile widths: 48^2, 100 steps."""
     run_pair(P.orszag_tang_2d(48), steps=100)
