@@ -37,7 +37,12 @@ using FSoa = SoaT<FT>;
 __host__ __device__ inline FSoa fs(const Soa& a) { return FSoa{reinterpret_cast<FT*>(a.p), a.stride}; }
 __host__ __device__ inline ET* et(double* p) { return reinterpret_cast<ET*>(p); }
 // A stored element widened to FP64 before any arithmetic
-template <class T> __device__ __forceinline__ double ed(const T* p, long i) { return double(p[i]); }
+// Element i of an array of the grid, addressed as the (uniform) base plus a
+// 32-bit byte offset: every array holds fewer than DG_MAX_ELEMS elements.
+template <class T> __device__ __forceinline__ const T& ld32(const T* p, long i) {
+    return *reinterpret_cast<const T*>(reinterpret_cast<const char*>(p) + (unsigned)((unsigned)i * (unsigned)sizeof(T)));
+}
+template <class T> __device__ __forceinline__ double ed(const T* p, long i) { return double(ld32(p, i)); }
 void ctu_hydro3(const StepCtx& c, hipStream_t s);
 
 #define KB __launch_bounds__(256)
@@ -575,9 +580,9 @@ __device__ __forceinline__ void cell_states_from(const SweepArgs& a, int i, int 
     long q = 0;
     if (MHD && D >= 2) {
         q = fe.at(i, j, k);
-        bx = (a.B[0][q + fe.s[0]] - a.B[0][q]) * a.dt_d[0];
-        by = (a.B[1][q + fe.s[1]] - a.B[1][q]) * a.dt_d[1];
-        if (D == 3) bz = (a.B[2][q + fe.s[2]] - a.B[2][q]) * a.dt_d[2];
+        bx = (ld32(a.B[0], q + fe.s[0]) - ld32(a.B[0], q)) * a.dt_d[0];
+        by = (ld32(a.B[1], q + fe.s[1]) - ld32(a.B[1], q)) * a.dt_d[1];
+        if (D == 3) bz = (ld32(a.B[2], q + fe.s[2]) - ld32(a.B[2], q)) * a.dt_d[2];
     }
     if (idx[AX] >= -g + 1 && idx[AX] < ce.n[AX] + g - 1) {
         if (AX == 0) {
@@ -596,7 +601,7 @@ __device__ __forceinline__ void cell_states_from(const SweepArgs& a, int i, int 
     }
     if (MHD && D >= 2) {  // copyFaceFields: normal B := face B
         const double* Bn = a.B[AX];
-        const R b0 = R(Bn[q]), b1 = R(Bn[q + fe.s[AX]]);
+        const R b0 = R(ld32(Bn, q)), b1 = R(ld32(Bn, q + fe.s[AX]));
         if (AX == 0) { L.Bx = b0; Rr.Bx = b1; }
         else if (AX == 1) { L.By = b0; Rr.By = b1; }
         else { L.Bz = b0; Rr.Bz = b1; }
@@ -1004,7 +1009,7 @@ __device__ __forceinline__ void faraday_body(const ET* Ex, const ET* Ey, const E
                                              const double* Biz, BO* Box_, BO* Boy, BO* Boz, const Ext& fe, int gg, double dt_dx,
                                              double dt_dy, double dt_dz, long m) {
     if (m >= fe.size) return;
-    double bx = Bix[m], by = Biy[m], bz = Biz[m];
+    double bx = ld32(Bix, m), by = ld32(Biy, m), bz = ld32(Biz, m);
     // recover (i,j,k) of m
     unsigned r = (unsigned)m;
     int i, j = 0, k = 0;
