@@ -95,7 +95,10 @@ typedef struct dg_stats {
 
 typedef struct dg_grid dg_grid;
 
-/* new GridND(n..., d..., ghosts) (Splitting.cpp:229-251) on the device. */
+/* new GridND(n..., d..., ghosts) (Splitting.cpp:229-251) on the device.
+ * Device code indexes each array in 32 bits: a grid whose face index space,
+ * (nx+1+2g)(ny+1+2g)(nz+1+2g), reaches 2^29 elements is refused with
+ * DG_ERR_ARG (on one GPU's memory only the largest 1D grids get there). */
 int dg_create(const dg_desc* desc, dg_grid** out);
 int dg_destroy(dg_grid* g);
 /* getGhosts() (Grid.hpp:54,79,121) */
