@@ -85,6 +85,8 @@ def test_struct_layout_matches_c(tmp_path):
 @pytest.mark.parametrize("bad", [
     dict(dim=0), dict(dim=4), dict(nx=0), dict(dim=2, ny=0), dict(dim=3, nz=0),
     dict(dx=0.0), dict(dim=2, dy=-1.0), dict(muscl=0, ctu=1),
+    dict(nx=(1 << 29) - 7),                                   # 1D face index space of 2^29 elements (32-bit indexing)
+    dict(dim=2, nx=30000, ny=20000), dict(dim=3, nx=900, ny=900, nz=900),
 ])
 def test_create_rejects_bad_descriptor(bad):
     """GridND constructors and CTU.cpp:26-30 reject these before any device
