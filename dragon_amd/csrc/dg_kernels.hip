@@ -78,6 +78,18 @@ template <class T> __device__ __forceinline__ U<double> load_u(const SoaT<T>& a,
     if (mhd) { u.Bx = a(5, m); u.By = a(6, m); u.Bz = a(7, m); } else { u.Bx = u.By = u.Bz = 0; }
     return u;
 }
+// An MHD flux record along axis AX with its normal-B component taken as 0
+// instead of loaded: every MHD solver returns it as exactly +0 (HLL, HLLE and
+// HLLD average the normal field of both states before the solve, and the
+// physical flux sets it to zero), so the sums it enters are unchanged.
+template <int AX, class T> __device__ __forceinline__ U<double> load_u_nb(const SoaT<T>& a, long m) {
+    U<double> u;
+    u.rho = a(0, m); u.E = a(1, m); u.mx = a(2, m); u.my = a(3, m); u.mz = a(4, m);
+    u.Bx = AX == 0 ? 0.0 : double(a(5, m));
+    u.By = AX == 1 ? 0.0 : double(a(6, m));
+    u.Bz = AX == 2 ? 0.0 : double(a(7, m));
+    return u;
+}
 template <class R, class T> __device__ __forceinline__ void store_u(const SoaT<T>& a, long m, const U<R>& u, bool mhd) {
     a(0, m) = T(u.rho); a(1, m) = T(u.E); a(2, m) = T(u.mx); a(3, m) = T(u.my); a(4, m) = T(u.mz);
     if (mhd) { a(5, m) = T(u.Bx); a(6, m) = T(u.By); a(7, m) = T(u.Bz); }
@@ -1080,9 +1092,9 @@ __device__ __forceinline__ void whalf_body(const Soa& W, const FSoa& FX, const F
     const long m = ce.at(i, j, k);
     P<double> w = load_p<double>(W, m, true);
     U<double> u = cons<true>(w, ph);
-    u = uadd<true>(u, umul<true>(usub<true>(load_u(FX, m, true), load_u(FX, m + ce.s[0], true)), hx));
-    u = uadd<true>(u, umul<true>(usub<true>(load_u(FY, m, true), load_u(FY, m + ce.s[1], true)), hy));
-    if (D == 3) u = uadd<true>(u, umul<true>(usub<true>(load_u(FZ, m, true), load_u(FZ, m + ce.s[2], true)), hz));
+    u = uadd<true>(u, umul<true>(usub<true>(load_u_nb<0>(FX, m), load_u_nb<0>(FX, m + ce.s[0])), hx));
+    u = uadd<true>(u, umul<true>(usub<true>(load_u_nb<1>(FY, m), load_u_nb<1>(FY, m + ce.s[1])), hy));
+    if (D == 3) u = uadd<true>(u, umul<true>(usub<true>(load_u_nb<2>(FZ, m), load_u_nb<2>(FZ, m + ce.s[2])), hz));
     if (!finite<true>(u)) report(err, ST_WHALF_NAN, (unsigned long long)m, E_NAN);
     w = prim<true>(u, ph);
     double cbx, cby, cbz;
@@ -1195,9 +1207,15 @@ __device__ __forceinline__ double apply_body(const Soa& W, const Soa& Wn, const 
     }
     P<double> w = load_p<double>(W, m, MHD);
     U<double> u = cons<MHD>(w, ph);
-    u = uadd<MHD>(u, umul<MHD>(usub<MHD>(load_u(FX, m, MHD), load_u(FX, m + ce.s[0], MHD)), dt_dx));
-    if (D >= 2) u = uadd<MHD>(u, umul<MHD>(usub<MHD>(load_u(FY, m, MHD), load_u(FY, m + ce.s[1], MHD)), dt_dy));
-    if (D == 3) u = uadd<MHD>(u, umul<MHD>(usub<MHD>(load_u(FZ, m, MHD), load_u(FZ, m + ce.s[2], MHD)), dt_dz));
+    if constexpr (MHD) {
+        u = uadd<true>(u, umul<true>(usub<true>(load_u_nb<0>(FX, m), load_u_nb<0>(FX, m + ce.s[0])), dt_dx));
+        if (D >= 2) u = uadd<true>(u, umul<true>(usub<true>(load_u_nb<1>(FY, m), load_u_nb<1>(FY, m + ce.s[1])), dt_dy));
+        if (D == 3) u = uadd<true>(u, umul<true>(usub<true>(load_u_nb<2>(FZ, m), load_u_nb<2>(FZ, m + ce.s[2])), dt_dz));
+    } else {
+        u = uadd<false>(u, umul<false>(usub<false>(load_u(FX, m, false), load_u(FX, m + ce.s[0], false)), dt_dx));
+        if (D >= 2) u = uadd<false>(u, umul<false>(usub<false>(load_u(FY, m, false), load_u(FY, m + ce.s[1], false)), dt_dy));
+        if (D == 3) u = uadd<false>(u, umul<false>(usub<false>(load_u(FZ, m, false), load_u(FZ, m + ce.s[2], false)), dt_dz));
+    }
     if (D >= 2 && !finite<MHD>(u)) report(err, ST_APPLY_NAN, (unsigned long long)m, E_NAN);
     w = prim<MHD>(u, ph);
     if (MHD && D >= 2) {
