@@ -938,14 +938,22 @@ KB __global__ void k_emf3(FSoa FX, FSoa FY, FSoa FZ, Soa W, ERef ref, Ext ce, Ex
 // sides; the previous plane stays in the other LDS buffer for the i-1
 // neighbours.  Every edge is evaluated by emf3_eval, as in k_emf3.
 constexpr int EMF_TK = 64, EMF_TJ = 4, EMF_CELLS = (EMF_TJ + 1) * (EMF_TK + 1);
-struct EmfPlane {
-    double r[3][EMF_CELLS];
-    FT f[3][3][EMF_CELLS];  // [axis][slot][cell]
+// emf3_eval reads the plane below the edge (offset a = -1) only through the
+// y and z fluxes' mass and first B slots and the reference EMF's y and z
+// components; those are kept for two planes (ring slot = plane & 1), the
+// rest (x fluxes, the y and z fluxes' second B slot, the x reference EMF)
+// for the current plane only.  Half the LDS of two full planes: more
+// workgroups per CU.
+struct EmfPlanes {
+    double rk[2][2][EMF_CELLS];  // reference EMF y, z; [ring][comp - 1][cell]
+    FT fk[2][4][EMF_CELLS];      // FY slots 0, 1 and FZ slots 0, 1; [ring][..][cell]
+    double rx[EMF_CELLS];        // reference EMF x, current plane
+    FT fo[5][EMF_CELLS];         // FX slots 0, 1, 2, FY slot 2, FZ slot 2, current plane
 };
 template <bool FROM_W>
 KB __global__ void k_emf3_tiled(FSoa FX, FSoa FY, FSoa FZ, Soa W, ERef ref, Ext ce, Ext fe, Box box, int upwind, int iplanes,
                                 ET* Ex, ET* Ey, ET* Ez) {
-    __shared__ EmfPlane pl[2];
+    __shared__ EmfPlanes S;
     const int nkt = (box.hi[2] - box.lo[2] + EMF_TK - 1) / EMF_TK;
     const int njt = (box.hi[1] - box.lo[1] + EMF_TJ - 1) / EMF_TJ;
     int b = blockIdx.x;
@@ -955,23 +963,33 @@ KB __global__ void k_emf3_tiled(FSoa FX, FSoa FY, FSoa FZ, Soa W, ERef ref, Ext 
     const int ic = b / njt;
     const int k0 = box.lo[2] + kt * EMF_TK, j0 = box.lo[1] + jt * EMF_TJ;
     const int ib = box.lo[0] + ic * iplanes, ie = min(box.hi[0], ib + iplanes);
-    auto load = [&](EmfPlane& P, int i) {
+    // plane i into LDS: the kept values into ring slot i & 1, the others (when `own`) into the current-plane arrays
+    auto load = [&](int i, bool own) {
+        const int rs = i & 1;
         for (int c = threadIdx.x; c < EMF_CELLS; c += blockDim.x) {
             const int j = j0 + c / (EMF_TK + 1) - 1, k = k0 + c % (EMF_TK + 1) - 1;
             if (j >= box.hi[1] || k >= box.hi[2]) continue;  // outside the box: no edge reads it
             const long q = ce.at(i, j, k);
-            for (int f = 0; f < 3; f++) {
-                P.f[0][f][c] = FX(emf_fld(0, f), q);
-                P.f[1][f][c] = FY(emf_fld(1, f), q);
-                P.f[2][f][c] = FZ(emf_fld(2, f), q);
+            S.fk[rs][0][c] = FY(emf_fld(1, 0), q);
+            S.fk[rs][1][c] = FY(emf_fld(1, 1), q);
+            S.fk[rs][2][c] = FZ(emf_fld(2, 0), q);
+            S.fk[rs][3][c] = FZ(emf_fld(2, 1), q);
+            if (own) {
+                for (int f = 0; f < 3; f++) S.fo[f][c] = FX(emf_fld(0, f), q);
+                S.fo[3][c] = FY(emf_fld(1, 2), q);
+                S.fo[4][c] = FZ(emf_fld(2, 2), q);
             }
             if (upwind) {
+                double x, y, z;
                 if (FROM_W) {
-                    cross(W(F_BX, q), W(F_BY, q), W(F_BZ, q), W(F_VX, q), W(F_VY, q), W(F_VZ, q), P.r[0][c], P.r[1][c], P.r[2][c]);
+                    cross(W(F_BX, q), W(F_BY, q), W(F_BZ, q), W(F_VX, q), W(F_VY, q), W(F_VZ, q), x, y, z);
                 } else {
                     const long e = fe.at(i, j, k);
-                    P.r[0][c] = ed(ref.x, e); P.r[1][c] = ed(ref.y, e); P.r[2][c] = ed(ref.z, e);
+                    x = ed(ref.x, e); y = ed(ref.y, e); z = ed(ref.z, e);
                 }
+                S.rk[rs][0][c] = y;
+                S.rk[rs][1][c] = z;
+                if (own) S.rx[c] = x;
             }
         }
     };
@@ -979,26 +997,23 @@ KB __global__ void k_emf3_tiled(FSoa FX, FSoa FY, FSoa FZ, Soa W, ERef ref, Ext 
     const int j = j0 + tj, k = k0 + tk;
     const bool live = j < box.hi[1] && k < box.hi[2];
     auto cell = [&](int dj, int dk) { return (tj + dj + 1) * (EMF_TK + 1) + (tk + dk + 1); };
-    int cur = 0;
-    load(pl[1], ib - 1);
+    load(ib - 1, false);
     for (int i = ib; i < ie; i++) {
-        load(pl[cur], i);
+        load(i, true);
         __syncthreads();
-        const EmfPlane& C = pl[cur];
-        const EmfPlane& Pv = pl[1 - cur];
-        auto pick = [&](int a) -> const EmfPlane& { return a < 0 ? Pv : C; };
-        auto fx = [&](int f, int a, int b2, int c2) { return double(pick(a).f[0][f][cell(b2, c2)]); };
-        auto fy = [&](int f, int a, int b2, int c2) { return double(pick(a).f[1][f][cell(b2, c2)]); };
-        auto fz = [&](int f, int a, int b2, int c2) { return double(pick(a).f[2][f][cell(b2, c2)]); };
-        auto R = [&](int comp, int a, int b2, int c2) { return pick(a).r[comp][cell(b2, c2)]; };
+        auto ring = [&](int a) { return (i + a) & 1; };  // a = 0 or -1
+        // the x fluxes and the reference EMF's x component are read at a = 0 only (emf3_eval)
+        auto fx = [&](int f, int, int b2, int c2) { return double(S.fo[f][cell(b2, c2)]); };
+        auto fy = [&](int f, int a, int b2, int c2) { return double(f < 2 ? S.fk[ring(a)][f][cell(b2, c2)] : S.fo[3][cell(b2, c2)]); };
+        auto fz = [&](int f, int a, int b2, int c2) { return double(f < 2 ? S.fk[ring(a)][2 + f][cell(b2, c2)] : S.fo[4][cell(b2, c2)]); };
+        auto R = [&](int comp, int a, int b2, int c2) { return comp == 0 ? S.rx[cell(b2, c2)] : S.rk[ring(a)][comp - 1][cell(b2, c2)]; };
         if (live) {
             double ex, ey, ez;
             emf3_eval(fx, fy, fz, R, upwind, ex, ey, ez);
             const long e = fe.at(i, j, k);
             Ex[e] = ex; Ey[e] = ey; Ez[e] = ez;
         }
-        __syncthreads();  // the previous plane's buffer is refilled next
-        cur = 1 - cur;
+        __syncthreads();  // the next plane overwrites the current-plane arrays and ring slot (i + 1) & 1
     }
 }
 
@@ -1491,9 +1506,6 @@ static void step_fused(const StepCtx& c, hipStream_t s) {
         if constexpr (D == 2)
             hipLaunchKernelGGL((k_emf2<false>), dim3(blocks(eb.count())), dim3(256), 0, s, fs(c.F[0]), fs(c.F[1]), c.W, eh, ce, c.fe, eb, c.ctu,
                                et(c.E0[0]), et(c.E0[1]), et(c.E0[2]));
-        else if constexpr (sizeof(FT) == 8)  // FP64 fluxes: the LDS planes allow 2 workgroups per CU, slower than k_emf3 here
-            hipLaunchKernelGGL((k_emf3<false>), dim3(blocks(eb.count())), dim3(256), 0, s, fs(c.F[0]), fs(c.F[1]), fs(c.F[2]), c.W, eh, ce, c.fe,
-                               eb, c.ctu, et(c.E0[0]), et(c.E0[1]), et(c.E0[2]));
         else
             emf3_tiled<false>(fs(c.F[0]), fs(c.F[1]), fs(c.F[2]), c.W, eh, ce, c.fe, eb, c.ctu, et(c.E0[0]), et(c.E0[1]), et(c.E0[2]), s);
         mark(c, K_EMF);
