@@ -99,10 +99,10 @@ def test_c3_orszag_tang_2048_to_t05():
     grids = [gpu_grid(pb, 0), gpu_grid(pb, 1)]
     start = [totals_2d(G, pb[0], True) for G in grids]
     t = 0.0
-    for t_frame, bar in ((0.25, 1e-5), (0.25, 2e-4)):
-        runs = [G.advance(t_frame) for G in grids]
+    for frame_len, bar in ((0.25, 1e-5), (0.25, 2e-4)):  # frames end at t = 0.25 and t = 0.5
+        runs = [G.advance(frame_len) for G in grids]
         now = [totals_2d(G, pb[0], True) for G in grids]
-        t += t_frame
+        t += frame_len
         for k, G in enumerate(grids):
             (t0, w0), (t1, w1) = start[k], now[k]
             scale = np.abs(w0[..., 0] * np.hypot(w0[..., 2], w0[..., 3])).sum()
