@@ -661,7 +661,14 @@ __device__ __forceinline__ void face_flux(const SweepArgs& a, P<R> l, P<R> r, lo
 template <class R> constexpr bool march_window() { return sizeof(R) == 4; }
 
 // Strided axis: one thread walks `chunk` consecutive faces.
-template <int D, bool MHD, class R, int AX, bool FINAL, bool WIN = march_window<R>()>
+template <class R> struct Pair2;
+template <> struct Pair2<float> { using T = float2; };
+template <> struct Pair2<double> { using T = double2; };
+
+// PARKALL: the windowed walker parks its carried state in LDS in the
+// predictor too (the batched walkers).  Without it, a must be the kernel's
+// own argument (the single-grid walkers).
+template <int D, bool MHD, class R, int AX, bool FINAL, bool WIN = march_window<R>(), bool PARKALL = false>
 __device__ __forceinline__ void sweep_march_body(const SweepArgs& a, long t) {
     constexpr int T0 = AX == 0 ? 1 : 0;            // slower transverse dim (3D only)
     constexpr int T1 = D == 3 ? (AX == 2 ? 1 : 2) : (AX == 0 ? 1 : 0);  // fastest transverse dim
@@ -687,21 +694,80 @@ __device__ __forceinline__ void sweep_march_body(const SweepArgs& a, long t) {
     long m = a.ce.at(idx[0], idx[1], idx[2]);
     auto ld = [&](int c, long mm) { return c >= clo && c < chi ? load_p<double>(a.W, mm, MHD) : pzero<double>(); };
     if constexpr (WIN) {
-        // The three cells of the MUSCL stencil stay in registers as the
-        // walker advances; the cell two ahead is read (as FP64) before the
-        // Riemann solve of the current face and narrowed when it is used.
-        P<R> wl = p_cast<R>(ld(f0 - 2, m - s)), wc = p_cast<R>(ld(f0 - 1, m));
+        // The three cells of the MUSCL stencil are carried as the walker
+        // advances; the cell two ahead is read (as FP64) before the Riemann
+        // solve of the current face and narrowed when it is used.  The
+        // predictor keeps the two cells behind it and the right state of the
+        // last cell in registers.  The corrector, which also carries the CTU
+        // corrections' operands, needs 159 registers (3 waves per SIMD) that
+        // way; where the walker's step is short (not the 3D x walker, whose
+        // step is a whole plane) it parks them in LDS instead, one slot per
+        // thread, and so do both batched walkers.  The parked single-grid
+        // corrector also reads its arguments again from the kernel-argument
+        // segment at each face, through an opaque copy of its address, so
+        // that no value derived from them (nor a constant the compiler would
+        // hoist with them) stays in a register across the loop: 116-118
+        // registers, 4 waves, where it would otherwise spill at 4.  (At 4
+        // waves the 3D x corrector ran 3% faster or 5% slower from run to
+        // run, the y corrector 8% faster in every run.)
+        constexpr bool PARK = (FINAL && !(D == 3 && AX == 0)) || PARKALL;
+        constexpr bool RELOAD = PARK && !PARKALL;
+        using R2 = typename Pair2<R>::T;
+        __shared__ R2 park[3 * 4 * 256];  // [slot][field pair][thread]: 256 threads per workgroup
+        R2* pk = park + threadIdx.x;
+        auto put = [&](int slot, const P<R>& w) {
+            R2* q = pk + slot * 1024;
+            q[0] = R2{w.rho, w.p}; q[256] = R2{w.vx, w.vy}; q[512] = R2{w.vz, w.Bx}; q[768] = R2{w.By, w.Bz};
+        };
+        auto get = [&](int slot) {
+            const R2* q = pk + slot * 1024;
+            const R2 a0 = q[0], a1 = q[256], a2 = q[512], a3 = q[768];
+            return P<R>{a0.x, a0.y, a1.x, a1.y, a2.x, a2.y, a3.x, a3.y};
+        };
+        P<R> wl = pzero<R>(), wc = pzero<R>();
+        if constexpr (PARK) {
+            put(0, p_cast<R>(ld(f0 - 2, m - s)));
+            put(1, p_cast<R>(ld(f0 - 1, m)));
+        } else {
+            wl = p_cast<R>(ld(f0 - 2, m - s));
+            wc = p_cast<R>(ld(f0 - 1, m));
+        }
         P<double> nxt = ld(f0, m + s);
-        for (int f = f0 - 1; f < f1; f++) {
+        int sl = 0;  // parked: slot of the stencil's left cell; its centre cell is in slot 1 - sl
+        typedef const __attribute__((address_space(4))) SweepArgs* KArg;
+        KArg kp = (KArg)__builtin_amdgcn_kernarg_segment_ptr();
+        auto step = [&](const SweepArgs& a, int f) {
             idx[AX] = f;
             const P<R> wr = p_cast<R>(nxt);
+            if constexpr (PARK) {
+                wl = get(sl);
+                wc = get(1 - sl);
+                put(sl, wr);  // the left cell's slot takes the new leading cell
+            }
             cell_states_from<D, MHD, R, AX, FINAL>(a, idx[0], idx[1], idx[2], wl, wc, wr, Lc, Rc);
-            nxt = ld(f + 2, m + 2 * s);
-            if (f >= f0) face_flux<MHD, R, AX>(a, Rp, Lc, m);  // uniform per wavefront
-            Rp = Rc;
-            wl = wc;
-            wc = wr;
+            nxt = f + 2 >= clo && f + 2 < chi ? load_p<double>(a.W, m + 2 * s, MHD) : pzero<double>();
+            if constexpr (PARK) {
+                if (f >= f0) Rp = get(2);
+                put(2, Rc);
+                // Rc's only copy for the next face is the one in LDS
+                asm volatile("" ::: "memory");
+                if (f >= f0) face_flux<MHD, R, AX>(a, Rp, Lc, m);  // uniform per wavefront
+                sl = 1 - sl;
+            } else {
+                if (f >= f0) face_flux<MHD, R, AX>(a, Rp, Lc, m);
+                Rp = Rc;
+                wl = wc;
+                wc = wr;
+            }
             m += s;
+        };
+        for (int f = f0 - 1; f < f1; f++) {
+            if constexpr (RELOAD) {
+                asm volatile("" : "+s"(kp));
+                step(*(const SweepArgs*)kp, f);
+            } else {
+                step(a, f);
+            }
         }
     } else {
         // The cell ahead of the stencil is read before the Riemann solve of
@@ -719,12 +785,14 @@ __device__ __forceinline__ void sweep_march_body(const SweepArgs& a, long t) {
         }
     }
 }
-// Occupancy floor of the walkers: the FP32 predictor walker needs 133
-// registers, one wave per SIMD short of four; at four it spills 8 bytes per
-// lane and runs 5-8% faster.  1 = the compiler's choice.
-template <class R, bool FINAL> constexpr int march_min_waves() { return DG_SWEEP_WAVES == 0 && sizeof(R) == 4 && !FINAL ? 4 : 1; }
+// Occupancy floor of the FP32 walkers: four waves per SIMD for the predictor
+// (133 registers, 8 bytes per lane spilled at four, 5-8% faster than three)
+// and the parked corrector (116-118 registers).  1 = the compiler's choice.
+template <int D, int AX, class R, bool FINAL> constexpr int march_min_waves() {
+    return DG_SWEEP_WAVES == 0 && sizeof(R) == 4 && (!FINAL || !(D == 3 && AX == 0)) ? 4 : 1;
+}
 template <int D, bool MHD, class R, int AX, bool FINAL>
-KS __attribute__((amdgpu_waves_per_eu(march_min_waves<R, FINAL>()))) __global__ void k_sweep_march(SweepArgs a) {
+KS __attribute__((amdgpu_waves_per_eu(march_min_waves<D, AX, R, FINAL>()))) __global__ void k_sweep_march(SweepArgs a) {
     sweep_march_body<D, MHD, R, AX, FINAL>(a, tid());
 }
 template <int D, bool MHD, class R, int AX, bool FINAL>
@@ -1836,7 +1904,7 @@ KS __global__ void kb_sweep_x(BatchArgs A, BlockTable bt) {
     SweepArgs s = batch_sweep_args<0, FIN>(A, v, p);
     const int len = s.hi[0] - s.lo[0];
     s.chunk = A.chunk[v.cls] < len ? A.chunk[v.cls] : len;
-    sweep_march_body<2, true, RT, 0, FIN>(s, t);
+    sweep_march_body<2, true, RT, 0, FIN, march_window<RT>(), true>(s, t);
 }
 template <bool FIN>
 KS __attribute__((amdgpu_waves_per_eu(wave_min_waves<RT>()))) __global__ void kb_sweep_y(BatchArgs A, BlockTable bt) {

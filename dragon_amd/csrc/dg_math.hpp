@@ -212,6 +212,10 @@ template <class T> __device__ __forceinline__ T limit1(T a, T b, int lim) {
 template <bool MHD, class T>
 __device__ __forceinline__ void muscl(const P<T>& wL, P<T>& L, const P<T>& wC, P<T>& R, const P<T>& wR, T dt_dL,
                                       T dBx, T dBy, T dBz, const Phys& ph) {
+    // The result is selected field by field: assigning whole records on
+    // the two paths left a private-memory copy in one instantiation.
+    bool ok = false;
+    P<T> l = wC, r = wC;
     if (ph.muscl) {
         const int lim = ph.limiter;
         P<T> dW;
@@ -225,7 +229,6 @@ __device__ __forceinline__ void muscl(const P<T>& wL, P<T>& L, const P<T>& wC, P
             dW.By = limit1(wC.By - wL.By, wR.By - wC.By, lim);
             dW.Bz = limit1(wC.Bz - wL.Bz, wR.Bz - wC.Bz, lim);
         }
-        P<T> l, r;
         l.rho = wC.rho - T(0.5) * dW.rho; r.rho = wC.rho + T(0.5) * dW.rho;
         l.vx = wC.vx - dW.vx * T(0.5); r.vx = wC.vx + dW.vx * T(0.5);
         l.vy = wC.vy - dW.vy * T(0.5); r.vy = wC.vy + dW.vy * T(0.5);
@@ -251,10 +254,10 @@ __device__ __forceinline__ void muscl(const P<T>& wL, P<T>& L, const P<T>& wC, P
         }
         l = prim<MHD>(usub<MHD>(UL, corr), ph);
         r = prim<MHD>(usub<MHD>(UR, corr), ph);
-        if (physical<MHD>(l) && physical<MHD>(r)) { L = l; R = r; return; }
+        ok = physical<MHD>(l) && physical<MHD>(r);
     }
-    L = wC;
-    R = wC;
+    L = psel(ok, l, wC);
+    R = psel(ok, r, wC);
 }
 
 // ---- HLL family (HLL.cpp)
