@@ -813,10 +813,11 @@ template <class R> __device__ __forceinline__ P<R> shfl_up1(const P<R>& w) {
                 __shfl_up(w.vz, 1), __shfl_up(w.Bx, 1), __shfl_up(w.By, 1), __shfl_up(w.Bz, 1)};
 }
 
-// Contiguous axis: lanes are grouped into segments of SW lanes (64, 32 or
-// 16); lane l of a segment rebuilds cell c0-1+l and lanes 1..SW-1 take the
-// left state from the lane below, so a segment computes SW-1 faces.  With
-// SW = 64 the segment is the wavefront and its index is wave-uniform.
+// Contiguous axis in segments (the batched y sweep): lanes are grouped into
+// segments of SW lanes (64, 32 or 16); lane l of a segment rebuilds cell
+// c0-1+l and lanes 1..SW-1 take the left state from the lane below, so a
+// segment computes SW-1 faces.  With SW = 64 the segment is the wavefront
+// and its index is wave-uniform.
 template <int D, bool MHD, class R, int AX, bool FINAL, int SW = 64>
 __device__ __forceinline__ void sweep_wave_body(const SweepArgs& a, long seg_id, int lane) {
     constexpr int T0 = 0;                 // slowest transverse dim
@@ -841,16 +842,43 @@ __device__ __forceinline__ void sweep_wave_body(const SweepArgs& a, long seg_id,
     const P<R> Rl = shfl_up1(Rr);  // lane 0 of a segment takes the lane below's value and leaves it unused
     if (ll >= 1 && live) face_flux<MHD, R, AX>(a, Rl, L, a.ce.at(idx[0], idx[1], idx[2]));
 }
+// Contiguous axis, streamed: the cells to rebuild, lo-1 .. hi-1 of every
+// line (line after line, which is also their order in memory when the range
+// spans the row), are dealt to wavefronts 63 at a time, each wavefront
+// taking one cell more than its 63 (the first cell of the next, rebuilt
+// twice).  Lane l >= 1 forms the face between its cell and lane l-1's unless
+// its cell is the first of a line.  Every face is formed once, and 63 of 64
+// lanes form a face except at line starts, where fixed segments of 63 faces
+// per line (9 wavefronts for the 517 faces of a 512^3 predictor line) left a
+// tenth of the lanes idle.
+template <int D, bool MHD, class R, int AX, bool FINAL>
+__device__ __forceinline__ void sweep_stream_body(const SweepArgs& a, int w, int lane) {
+    static_assert(AX == D - 1, "contiguous axis");
+    const int len = a.hi[AX] - a.lo[AX];
+    const int cells = len + 1;  // per line
+    const int n1 = D == 3 ? a.hi[1] - a.lo[1] : 1;
+    const int n0 = D >= 2 ? a.hi[0] - a.lo[0] : 1;
+    const unsigned g = 63u * (unsigned)w + (unsigned)lane;  // < 2^31: the host sizes grids far below
+    const bool live = g < (unsigned)(n0 * n1 * cells);
+    const unsigned line = g / (unsigned)cells, c = g - line * (unsigned)cells;
+    int idx[3] = {0, 0, 0};
+    if (D == 3) { idx[0] = a.lo[0] + (int)(line / (unsigned)n1); idx[1] = a.lo[1] + (int)(line % (unsigned)n1); }
+    if (D == 2) idx[0] = a.lo[0] + (int)line;
+    idx[AX] = a.lo[AX] - 1 + (int)c;
+    P<R> L = pzero<R>(), Rr = pzero<R>();
+    if (live) cell_states<D, MHD, R, AX, FINAL>(a, idx[0], idx[1], idx[2], L, Rr);
+    const P<R> Rl = shfl_up1(Rr);  // lane 0 takes the lane below's value and leaves it unused
+    if (lane >= 1 && live && c >= 1) face_flux<MHD, R, AX>(a, Rl, L, a.ce.at(idx[0], idx[1], idx[2]));
+}
 // Occupancy floor of the contiguous-axis sweeps: the FP64 ones need 176
 // registers, 8 short of three waves per SIMD; at three they spill 12 bytes
 // per lane.  1 = the compiler's choice.
 template <class R> constexpr int wave_min_waves() { return DG_SWEEP_WAVES == 0 && sizeof(R) == 8 ? 3 : 1; }
 template <int D, bool MHD, class R, int AX, bool FINAL>
 KS __attribute__((amdgpu_waves_per_eu(wave_min_waves<R>()))) __global__ void k_sweep_wave(SweepArgs a) {
-    // the wavefront index is uniform: read it into a scalar register so the
-    // index arithmetic of sweep_wave_body runs once per wavefront on the SALU
+    // the wavefront index is uniform: keep it in a scalar register
     const int w = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
-    sweep_wave_body<D, MHD, R, AX, FINAL>(a, (long)blockIdx.x * (blockDim.x >> 6) + w, threadIdx.x & 63);
+    sweep_stream_body<D, MHD, R, AX, FINAL>(a, (int)blockIdx.x * (int)(blockDim.x >> 6) + w, threadIdx.x & 63);
 }
 
 // ===================================================== CTU hydro (2D/3D)
@@ -1507,8 +1535,9 @@ static void sweep_t(const StepCtx& c, const Box& b, const FSoa& out, int stage, 
     for (int d = 0; d < D; d++)
         if (d != AX) ntrans *= b.hi[d] - b.lo[d];
     if (len <= 0 || ntrans <= 0) return;
-    if (AX == D - 1) {
-        const long waves = ((len + 62) / 63) * ntrans;
+    if constexpr (AX == D - 1) {
+        const long cells = ntrans * (len + 1);
+        const long waves = cells > 1 ? (cells - 1 + 62) / 63 : 1;
         a.chunk = 63;
         hipLaunchKernelGGL((k_sweep_wave<D, MHD, R, AX, FINAL>), dim3(blocks(waves * 64)), dim3(256), 0, s, a);
     } else {
