@@ -670,14 +670,15 @@ __device__ __forceinline__ void face_flux(const SweepArgs& a, P<R> l, P<R> r, lo
 // window's cell bookkeeping: its results must equal the oracle's bit for bit.
 template <class R> constexpr bool march_window() { return sizeof(R) == 4; }
 
-// Strided axis: one thread walks `chunk` consecutive faces.
+// Two values of the walker's precision, one LDS access.
 template <class R> struct Pair2;
 template <> struct Pair2<float> { using T = float2; };
 template <> struct Pair2<double> { using T = double2; };
 
-// PARKALL: the windowed walker parks its carried state in LDS in the
-// predictor too (the batched walkers).  Without it, a must be the kernel's
-// own argument (the single-grid walkers).
+// Strided axis: one thread walks `chunk` consecutive faces.  PARKALL: the
+// windowed walker parks its carried state in LDS in the predictor too (the
+// batched walkers); without it, a must be the kernel's own argument (the
+// single-grid walkers).
 template <int D, bool MHD, class R, int AX, bool FINAL, bool WIN = march_window<R>(), bool PARKALL = false>
 __device__ __forceinline__ void sweep_march_body(const SweepArgs& a, long t) {
     constexpr int T0 = AX == 0 ? 1 : 0;            // slower transverse dim (3D only)
