@@ -700,7 +700,8 @@ class Batch:
         self._chk(lib().dg_batch_set_tiles(self._h, arr, len(chunks)))
 
     def ysweep(self):
-        """The y sweep's segment width (lanes) per size class."""
+        """The y sweep's segment width (lanes) per size class; 0: each
+        patch's columns streamed through its wavefronts."""
         out = (C.c_int * BATCH_CLASSES)()
         self._chk(lib().dg_batch_get_ysweep(self._h, out, BATCH_CLASSES))
         return list(out)

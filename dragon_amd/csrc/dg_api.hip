@@ -1026,8 +1026,13 @@ static long batch_items(const dg_batch* b, const PatchDesc& q, int kind) {
         const long c = std::min<long>(b->chunk[q.cls], len);
         return ((len + c - 1) / c) * rows;
     };
-    auto segments = [&](long len, long rows) {  // whole wavefronts of SW-lane segments
-        const long sw = b->ysw[q.cls], segs = ((len + sw - 2) / (sw - 1)) * rows;
+    auto segments = [&](long len, long rows) {  // whole wavefronts of SW-lane segments, or the cells streamed (SW 0)
+        const long sw = b->ysw[q.cls];
+        if (sw == 0) {
+            const long cells = rows * (len + 1);
+            return (cells > 1 ? (cells - 1 + 62) / 63 : 1) * 64;
+        }
+        const long segs = ((len + sw - 2) / (sw - 1)) * rows;
         return (segs * sw + 63) / 64 * 64;
     };
     switch (kind) {
@@ -1311,7 +1316,7 @@ int dg_batch_get_tiles(const dg_batch* b, int* chunk, int n) {
 int dg_batch_autotune(dg_batch* b, int reps, int* chunk_out, double* class_ms) {
     if (!b || reps < 1) return DG_ERR_ARG;
     static const int cand[] = {2, 4, 8, 16, 32, 64, 128};
-    static const int ycand[] = {64, 32, 16};
+    static const int ycand[] = {64, 32, 16, 0};
     std::vector<int> ncls(BATCH_CLASSES, 0);
     for (const auto& q : b->P) ncls[q.cls]++;
     BCHK(hipStreamSynchronize(b->s));
@@ -1362,7 +1367,7 @@ int dg_batch_autotune(dg_batch* b, int reps, int* chunk_out, double* class_ms) {
 int dg_batch_set_ysweep(dg_batch* b, const int* width, int n) {
     if (!b || !width || n < 1 || n > BATCH_CLASSES) return DG_ERR_ARG;
     for (int c = 0; c < n; c++)
-        if (width[c] != 64 && width[c] != 32 && width[c] != 16) return DG_ERR_ARG;
+        if (width[c] != 64 && width[c] != 32 && width[c] != 16 && width[c] != 0) return DG_ERR_ARG;
     BCHK(hipStreamSynchronize(b->s));
     batch_drop_graph(b);
     for (int c = 0; c < n; c++) b->ysw[c] = width[c];

@@ -1954,6 +1954,7 @@ KS __attribute__((amdgpu_waves_per_eu(wave_min_waves<RT>()))) __global__ void kb
     const long w = (long)bt.first[blockIdx.x] / 64 + __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
     const int lane = threadIdx.x & 63;
     switch (A.ysw[v.cls]) {  // segment width of the patch's size class (uniform per workgroup)
+        case 0: s.chunk = 63; sweep_stream_body<2, true, RT, 1, FIN>(s, (int)w, lane); break;  // the patch's columns streamed
         case 16: s.chunk = 15; sweep_wave_body<2, true, RT, 1, FIN, 16>(s, w * 4 + lane / 16, lane); break;
         case 32: s.chunk = 31; sweep_wave_body<2, true, RT, 1, FIN, 32>(s, w * 2 + lane / 32, lane); break;
         default: s.chunk = 63; sweep_wave_body<2, true, RT, 1, FIN, 64>(s, w, lane); break;

@@ -262,7 +262,9 @@ int dg_batch_patch_stats(dg_batch* b, int patch, double* dt_last, long* steps, l
  * per class: the number of x faces one thread walks in the x sweep (tiles),
  * and the y sweep's segment width, 64, 32 or 16 lanes (one, two or four
  * segments of 63, 31 or 15 faces per wavefront; narrow patches waste fewer
- * lanes with short segments).  autotune times each candidate on the class's
+ * lanes with short segments), or 0: the patch's columns streamed through its
+ * wavefronts, 63 cells each, with no segment boundary inside a column.
+ * autotune times each candidate on the class's
  * patches (steps not committed), the x tile first, then the y width, and
  * keeps the fastest; class_ms (optional) gets the best time per launch.
  * Results do not depend on either shape. */
