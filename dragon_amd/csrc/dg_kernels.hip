@@ -1039,75 +1039,130 @@ KB __global__ void k_emf3(FSoa FX, FSoa FY, FSoa FZ, Soa W, ERef ref, Ext ce, Ex
 }
 
 // The same edges blocked for reuse: a workgroup owns a (j,k) tile of
-// EMF_TJ x EMF_TK edges and walks a run of i-planes.  Each cell's operands
-// (9 flux components and its reference EMF, computed once) are read from
-// memory once per plane into LDS, with a one-cell halo on the low j and k
-// sides; the previous plane stays in the other LDS buffer for the i-1
-// neighbours.  Every edge is evaluated by emf3_eval, as in k_emf3.
-constexpr int EMF_TK = 64, EMF_TJ = 4, EMF_CELLS = (EMF_TJ + 1) * (EMF_TK + 1);
+// TJ x TK edges and walks a run of i-planes.  Each cell's operands (9 flux
+// components and its reference EMF, computed once) are read from memory once
+// per plane into LDS, with a one-cell halo on the low j and k sides; the
+// previous plane stays in the other LDS buffer for the i-1 neighbours.  Every
+// edge is evaluated by emf3_eval, as in k_emf3.
+//
+// Two tile shapes.  With PF the tile and its halo are one cell per thread
+// (3 x 63 edges, 4 x 64 cells), and each thread reads the next plane's cell
+// into registers before the current plane's edges are evaluated, so the
+// reads' latency overlaps that work.  Without it the tile is 4 x 64 edges
+// (5 x 65 cells, fewer halo reads per edge) and each plane is read, then
+// evaluated: the predictor EMF of the FP32 form, which reads the cells' B and
+// v in FP64 to form the reference EMF, is bound by those bytes and runs 3%
+// faster so; the other three instances run 4-12% faster with PF.
+template <bool PF> struct EmfTile {
+    static constexpr int TK = PF ? 63 : 64, TJ = PF ? 3 : 4, CELLS = (TJ + 1) * (TK + 1);
+};
+static_assert(EmfTile<true>::CELLS == 256, "one halo-extended tile cell per thread");
+template <bool FROM_W> constexpr bool emf_prefetch() { return !(FROM_W && sizeof(FT) == 4); }
 // emf3_eval reads the plane below the edge (offset a = -1) only through the
 // y and z fluxes' mass and first B slots and the reference EMF's y and z
 // components; those are kept for two planes (ring slot = plane & 1), the
 // rest (x fluxes, the y and z fluxes' second B slot, the x reference EMF)
 // for the current plane only.  Half the LDS of two full planes: more
 // workgroups per CU.
-struct EmfPlanes {
-    double rk[2][2][EMF_CELLS];  // reference EMF y, z; [ring][comp - 1][cell]
-    FT fk[2][4][EMF_CELLS];      // FY slots 0, 1 and FZ slots 0, 1; [ring][..][cell]
-    double rx[EMF_CELLS];        // reference EMF x, current plane
-    FT fo[5][EMF_CELLS];         // FX slots 0, 1, 2, FY slot 2, FZ slot 2, current plane
+template <int CELLS> struct EmfPlanes {
+    double rk[2][2][CELLS];  // reference EMF y, z; [ring][comp - 1][cell]
+    FT fk[2][4][CELLS];      // FY slots 0, 1 and FZ slots 0, 1; [ring][..][cell]
+    double rx[CELLS];        // reference EMF x, current plane
+    FT fo[5][CELLS];         // FX slots 0, 1, 2, FY slot 2, FZ slot 2, current plane
 };
+// FP32 operands: five workgroups per CU fit in LDS (23-30 KB each), and at
+// most 96 registers keep five waves per SIMD for them.
+template <class T> constexpr int emf_tiled_waves() { return sizeof(T) == 4 ? 5 : 1; }
 template <bool FROM_W>
-KB __global__ void k_emf3_tiled(FSoa FX, FSoa FY, FSoa FZ, Soa W, ERef ref, Ext ce, Ext fe, Box box, int upwind, int iplanes,
-                                ET* Ex, ET* Ey, ET* Ez) {
-    __shared__ EmfPlanes S;
-    const int nkt = (box.hi[2] - box.lo[2] + EMF_TK - 1) / EMF_TK;
-    const int njt = (box.hi[1] - box.lo[1] + EMF_TJ - 1) / EMF_TJ;
+KB __attribute__((amdgpu_waves_per_eu(emf_tiled_waves<FT>()))) __global__ void k_emf3_tiled(
+    FSoa FX, FSoa FY, FSoa FZ, Soa W, ERef ref, Ext ce, Ext fe, Box box, int upwind, int iplanes, ET* Ex, ET* Ey, ET* Ez) {
+    constexpr bool PF = emf_prefetch<FROM_W>();
+    constexpr int TK = EmfTile<PF>::TK, TJ = EmfTile<PF>::TJ, CELLS = EmfTile<PF>::CELLS;
+    __shared__ EmfPlanes<CELLS> S;
+    const int nkt = (box.hi[2] - box.lo[2] + TK - 1) / TK;
+    const int njt = (box.hi[1] - box.lo[1] + TJ - 1) / TJ;
     int b = blockIdx.x;
     const int kt = b % nkt;
     b /= nkt;
     const int jt = b % njt;
     const int ic = b / njt;
-    const int k0 = box.lo[2] + kt * EMF_TK, j0 = box.lo[1] + jt * EMF_TJ;
+    const int k0 = box.lo[2] + kt * TK, j0 = box.lo[1] + jt * TJ;
     const int ib = box.lo[0] + ic * iplanes, ie = min(box.hi[0], ib + iplanes);
-    // plane i into LDS: the kept values into ring slot i & 1, the others (when `own`) into the current-plane arrays
-    auto load = [&](int i, bool own) {
-        const int rs = i & 1;
-        for (int c = threadIdx.x; c < EMF_CELLS; c += blockDim.x) {
-            const int j = j0 + c / (EMF_TK + 1) - 1, k = k0 + c % (EMF_TK + 1) - 1;
-            if (j >= box.hi[1] || k >= box.hi[2]) continue;  // outside the box: no edge reads it
-            const long q = ce.at(i, j, k);
-            S.fk[rs][0][c] = FY(emf_fld(1, 0), q);
-            S.fk[rs][1][c] = FY(emf_fld(1, 1), q);
-            S.fk[rs][2][c] = FZ(emf_fld(2, 0), q);
-            S.fk[rs][3][c] = FZ(emf_fld(2, 1), q);
-            if (own) {
-                for (int f = 0; f < 3; f++) S.fo[f][c] = FX(emf_fld(0, f), q);
-                S.fo[3][c] = FY(emf_fld(1, 2), q);
-                S.fo[4][c] = FZ(emf_fld(2, 2), q);
-            }
-            if (upwind) {
-                double x, y, z;
-                if (FROM_W) {
-                    cross(W(F_BX, q), W(F_BY, q), W(F_BZ, q), W(F_VX, q), W(F_VY, q), W(F_VZ, q), x, y, z);
-                } else {
-                    const long e = fe.at(i, j, k);
-                    x = ed(ref.x, e); y = ed(ref.y, e); z = ed(ref.z, e);
-                }
-                S.rk[rs][0][c] = y;
-                S.rk[rs][1][c] = z;
-                if (own) S.rx[c] = x;
+    // a cell c of a plane: read into registers (fetch), then into LDS
+    // (stash): the kept values into ring slot i & 1, the others (when `own`)
+    // into the current-plane arrays.  Cells outside the box are skipped: no
+    // edge reads them.
+    struct CellVals {
+        FT f[9];      // FY slots 0, 1, FZ slots 0, 1, FX slots 0, 1, 2, FY slot 2, FZ slot 2
+        double w[6];  // B and v of the cell (FROM_W), or the reference EMF's x, y, z
+    };
+    auto in_box = [&](int c) { return j0 + c / (TK + 1) - 1 < box.hi[1] && k0 + c % (TK + 1) - 1 < box.hi[2]; };
+    auto fetch = [&](int c, int i, bool own, CellVals& v) {
+        const int cj = j0 + c / (TK + 1) - 1, ck = k0 + c % (TK + 1) - 1;
+        const long q = ce.at(i, cj, ck);
+        v.f[0] = FY(emf_fld(1, 0), q);
+        v.f[1] = FY(emf_fld(1, 1), q);
+        v.f[2] = FZ(emf_fld(2, 0), q);
+        v.f[3] = FZ(emf_fld(2, 1), q);
+        if (own) {
+            for (int f = 0; f < 3; f++) v.f[4 + f] = FX(emf_fld(0, f), q);
+            v.f[7] = FY(emf_fld(1, 2), q);
+            v.f[8] = FZ(emf_fld(2, 2), q);
+        }
+        if (upwind) {
+            if (FROM_W) {
+                v.w[0] = W(F_BX, q); v.w[1] = W(F_BY, q); v.w[2] = W(F_BZ, q);
+                v.w[3] = W(F_VX, q); v.w[4] = W(F_VY, q); v.w[5] = W(F_VZ, q);
+            } else {
+                const long e = fe.at(i, cj, ck);
+                v.w[0] = ed(ref.x, e); v.w[1] = ed(ref.y, e); v.w[2] = ed(ref.z, e);
             }
         }
     };
-    const int tk = threadIdx.x % EMF_TK, tj = threadIdx.x / EMF_TK;
+    auto stash = [&](int c, int i, bool own, const CellVals& v) {
+        const int rs = i & 1;
+        for (int f = 0; f < 4; f++) S.fk[rs][f][c] = v.f[f];
+        if (own)
+            for (int f = 0; f < 5; f++) S.fo[f][c] = v.f[4 + f];
+        if (upwind) {
+            double x = v.w[0], y = v.w[1], z = v.w[2];
+            if (FROM_W) cross(v.w[0], v.w[1], v.w[2], v.w[3], v.w[4], v.w[5], x, y, z);
+            S.rk[rs][0][c] = y;
+            S.rk[rs][1][c] = z;
+            if (own) S.rx[c] = x;
+        }
+    };
+    auto load = [&](int i, bool own) {  // the whole plane at once (no prefetch)
+        for (int c = threadIdx.x; c < CELLS; c += blockDim.x) {
+            if (!in_box(c)) continue;
+            CellVals v;
+            fetch(c, i, own, v);
+            stash(c, i, own, v);
+        }
+    };
+    const int tk = threadIdx.x % TK, tj = threadIdx.x / TK;
     const int j = j0 + tj, k = k0 + tk;
-    const bool live = j < box.hi[1] && k < box.hi[2];
-    auto cell = [&](int dj, int dk) { return (tj + dj + 1) * (EMF_TK + 1) + (tk + dk + 1); };
-    load(ib - 1, false);
+    const bool live = tj < TJ && j < box.hi[1] && k < box.hi[2];
+    auto cell = [&](int dj, int dk) { return (tj + dj + 1) * (TK + 1) + (tk + dk + 1); };
+    const int mc = threadIdx.x;  // PF: this thread's cell
+    const bool mine = PF && in_box(mc);
+    CellVals v;
+    if constexpr (PF) {
+        if (mine) { fetch(mc, ib - 1, false, v); stash(mc, ib - 1, false, v); }
+        if (mine) fetch(mc, ib, true, v);
+    } else {
+        load(ib - 1, false);
+    }
     for (int i = ib; i < ie; i++) {
-        load(i, true);
+        if constexpr (PF) {
+            if (mine) stash(mc, i, true, v);
+        } else {
+            load(i, true);
+        }
         __syncthreads();
+        if constexpr (PF) {
+            if (mine && i + 1 < ie) fetch(mc, i + 1, true, v);  // waited for at the next stash, after this plane's edges
+        }
         auto ring = [&](int a) { return (i + a) & 1; };  // a = 0 or -1
         // the x fluxes and the reference EMF's x component are read at a = 0 only (emf3_eval)
         auto fx = [&](int f, int, int b2, int c2) { return double(S.fo[f][cell(b2, c2)]); };
@@ -1129,7 +1184,8 @@ constexpr int EMF_IPLANES = 32;
 template <bool FROM_W>
 static void emf3_tiled(const FSoa& FX, const FSoa& FY, const FSoa& FZ, const Soa& W, const ERef& ref, const Ext& ce, const Ext& fe,
                        const Box& eb, int upwind, ET* Ex, ET* Ey, ET* Ez, hipStream_t s) {
-    const long nkt = (eb.hi[2] - eb.lo[2] + EMF_TK - 1) / EMF_TK, njt = (eb.hi[1] - eb.lo[1] + EMF_TJ - 1) / EMF_TJ;
+    using T = EmfTile<emf_prefetch<FROM_W>()>;
+    const long nkt = (eb.hi[2] - eb.lo[2] + T::TK - 1) / T::TK, njt = (eb.hi[1] - eb.lo[1] + T::TJ - 1) / T::TJ;
     const long nic = (eb.hi[0] - eb.lo[0] + EMF_IPLANES - 1) / EMF_IPLANES;
     if (nkt <= 0 || njt <= 0 || nic <= 0) return;
     hipLaunchKernelGGL((k_emf3_tiled<FROM_W>), dim3((unsigned)(nkt * njt * nic)), dim3(256), 0, s, FX, FY, FZ, W, ref, ce, fe, eb, upwind,
