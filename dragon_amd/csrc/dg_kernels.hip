@@ -473,7 +473,15 @@ struct SweepArgs {
     Phys ph;
     unsigned long long* err;
     int stage, chunk;
+    ET* eref[3];           // FP32 form, 3D MHD predictor z sweep: the cells' reference EMF cross(B, v) (else unused)
 };
+// The FP32 form's 3D MHD predictor z sweep also writes each rebuilt cell's
+// reference EMF, cross(B, v) of the state, rounded to FP32: its cells cover
+// the predictor EMF's ([-2, n+1] in every dimension), which then reads 12
+// bytes per cell instead of B and v in FP64 (48 bytes).
+template <int D, bool MHD, bool FINAL, int AX> constexpr bool eref_in_sweep() {
+    return D == 3 && MHD && !FINAL && AX == 2 && sizeof(FT) == 4;
+}
 
 // Conversions of the CTU-corrected half states.  The FP64 build keeps the
 // reference's divisions.  The mixed build narrows these states to FP32 right
@@ -868,6 +876,14 @@ __device__ __forceinline__ void sweep_stream_body(const SweepArgs& a, int w, int
     idx[AX] = a.lo[AX] - 1 + (int)c;
     P<R> L = pzero<R>(), Rr = pzero<R>();
     if (live) cell_states<D, MHD, R, AX, FINAL>(a, idx[0], idx[1], idx[2], L, Rr);
+    if constexpr (eref_in_sweep<D, MHD, FINAL, AX>()) {
+        if (live && a.eref[0]) {  // a cell rebuilt by two wavefronts is written twice, with the same bits
+            const long m = a.ce.at(idx[0], idx[1], idx[2]), e = a.fe.at(idx[0], idx[1], idx[2]);
+            double x, y, z;
+            cross(a.W(F_BX, m), a.W(F_BY, m), a.W(F_BZ, m), a.W(F_VX, m), a.W(F_VY, m), a.W(F_VZ, m), x, y, z);
+            a.eref[0][e] = ET(x); a.eref[1][e] = ET(y); a.eref[2][e] = ET(z);
+        }
+    }
     const P<R> Rl = shfl_up1(Rr);  // lane 0 takes the lane below's value and leaves it unused
     if (lane >= 1 && live && c >= 1) face_flux<MHD, R, AX>(a, Rl, L, a.ce.at(idx[0], idx[1], idx[2]));
 }
@@ -1587,6 +1603,9 @@ static void sweep_t(const StepCtx& c, const Box& b, const FSoa& out, int stage, 
         a.dt_d[q] = c.dt / c.d[q];
     }
     a.Fo = out; a.ce = c.ce; a.fe = c.fe; a.ph = c.ph; a.err = c.err; a.stage = stage;
+    if constexpr (eref_in_sweep<D, MHD, FINAL, AX>())
+        if (stage == ST_PRED_X + AX)  // the CTU predictor (not the flux sweep of a step without CTU)
+            for (int q = 0; q < 3; q++) a.eref[q] = et(c.Eh[q]);  // read by the predictor EMF before whalf overwrites it
     const long len = b.hi[AX] - b.lo[AX];
     long ntrans = 1;
     for (int d = 0; d < D; d++)
@@ -1645,6 +1664,9 @@ static void step_fused(const StepCtx& c, hipStream_t s) {
             if constexpr (D == 2)
                 hipLaunchKernelGGL((k_emf2<true>), dim3(blocks(eb.count())), dim3(256), 0, s, fs(c.Fp[0]), fs(c.Fp[1]), c.W, none, ce, c.fe, eb, 1,
                                    et(c.E0[0]), et(c.E0[1]), et(c.E0[2]));
+            else if constexpr (eref_in_sweep<D, MHD, false, D - 1>())  // the reference EMF from the z sweep
+                emf3_tiled<false>(fs(c.Fp[0]), fs(c.Fp[1]), fs(c.Fp[2]), c.W, ERef{et(c.Eh[0]), et(c.Eh[1]), et(c.Eh[2])}, ce, c.fe, eb, 1,
+                                  et(c.E0[0]), et(c.E0[1]), et(c.E0[2]), s);
             else
                 emf3_tiled<true>(fs(c.Fp[0]), fs(c.Fp[1]), fs(c.Fp[2]), c.W, none, ce, c.fe, eb, 1, et(c.E0[0]), et(c.E0[1]), et(c.E0[2]), s);
             mark(c, K_EMF_PRED);
