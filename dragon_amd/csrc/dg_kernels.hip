@@ -475,12 +475,22 @@ struct SweepArgs {
     int stage, chunk;
     ET* eref[3];           // FP32 form, 3D MHD predictor z sweep: the cells' reference EMF cross(B, v) (else unused)
 };
-// The FP32 form's 3D MHD predictor z sweep also writes each rebuilt cell's
-// reference EMF, cross(B, v) of the state, rounded to FP32: its cells cover
-// the predictor EMF's ([-2, n+1] in every dimension), which then reads 12
-// bytes per cell instead of B and v in FP64 (48 bytes).
+// The FP32 form's MHD predictor sweep along the contiguous axis also writes
+// each rebuilt cell's reference EMF, cross(B, v) of the state, rounded to
+// FP32: its cells cover the predictor EMF's ([-2, n+1] in every dimension),
+// which then reads 12 bytes per cell instead of B and v in FP64 (48 bytes),
+// and in 2D forms each cell's once instead of once per corner.
 template <int D, bool MHD, bool FINAL, int AX> constexpr bool eref_in_sweep() {
-    return D == 3 && MHD && !FINAL && AX == 2 && sizeof(FT) == 4;
+    return D >= 2 && AX == D - 1 && MHD && !FINAL && sizeof(FT) == 4;
+}
+// A cell rebuilt by two lanes (segment or wavefront overlaps) is written
+// twice, with the same bits.
+__device__ __forceinline__ void store_eref(const SweepArgs& a, bool live, const int* idx) {
+    if (!live || !a.eref[0]) return;
+    const long m = a.ce.at(idx[0], idx[1], idx[2]), e = a.fe.at(idx[0], idx[1], idx[2]);
+    double x, y, z;
+    cross(a.W(F_BX, m), a.W(F_BY, m), a.W(F_BZ, m), a.W(F_VX, m), a.W(F_VY, m), a.W(F_VZ, m), x, y, z);
+    a.eref[0][e] = ET(x); a.eref[1][e] = ET(y); a.eref[2][e] = ET(z);
 }
 
 // Conversions of the CTU-corrected half states.  The FP64 build keeps the
@@ -848,6 +858,7 @@ __device__ __forceinline__ void sweep_wave_body(const SweepArgs& a, long seg_id,
     P<R> L = pzero<R>(), Rr = pzero<R>();
     idx[AX] = c;
     if (live) cell_states<D, MHD, R, AX, FINAL>(a, idx[0], idx[1], idx[2], L, Rr);
+    if constexpr (eref_in_sweep<D, MHD, FINAL, AX>()) store_eref(a, live, idx);
     const P<R> Rl = shfl_up1(Rr);  // lane 0 of a segment takes the lane below's value and leaves it unused
     if (ll >= 1 && live) face_flux<MHD, R, AX>(a, Rl, L, a.ce.at(idx[0], idx[1], idx[2]));
 }
@@ -876,14 +887,7 @@ __device__ __forceinline__ void sweep_stream_body(const SweepArgs& a, int w, int
     idx[AX] = a.lo[AX] - 1 + (int)c;
     P<R> L = pzero<R>(), Rr = pzero<R>();
     if (live) cell_states<D, MHD, R, AX, FINAL>(a, idx[0], idx[1], idx[2], L, Rr);
-    if constexpr (eref_in_sweep<D, MHD, FINAL, AX>()) {
-        if (live && a.eref[0]) {  // a cell rebuilt by two wavefronts is written twice, with the same bits
-            const long m = a.ce.at(idx[0], idx[1], idx[2]), e = a.fe.at(idx[0], idx[1], idx[2]);
-            double x, y, z;
-            cross(a.W(F_BX, m), a.W(F_BY, m), a.W(F_BZ, m), a.W(F_VX, m), a.W(F_VY, m), a.W(F_VZ, m), x, y, z);
-            a.eref[0][e] = ET(x); a.eref[1][e] = ET(y); a.eref[2][e] = ET(z);
-        }
-    }
+    if constexpr (eref_in_sweep<D, MHD, FINAL, AX>()) store_eref(a, live, idx);
     const P<R> Rl = shfl_up1(Rr);  // lane 0 takes the lane below's value and leaves it unused
     if (lane >= 1 && live && c >= 1) face_flux<MHD, R, AX>(a, Rl, L, a.ce.at(idx[0], idx[1], idx[2]));
 }
@@ -1661,7 +1665,10 @@ static void step_fused(const StepCtx& c, hipStream_t s) {
         if constexpr (MHD) {
             ERef none{nullptr, nullptr, nullptr};
             Box eb = make_box(D, -1, ce.n[0] + 2, -1, ce.n[1] + 2, -1, ce.n[2] + 2);
-            if constexpr (D == 2)
+            if constexpr (D == 2 && eref_in_sweep<D, MHD, false, D - 1>())  // the reference EMF from the y sweep
+                hipLaunchKernelGGL((k_emf2<false>), dim3(blocks(eb.count())), dim3(256), 0, s, fs(c.Fp[0]), fs(c.Fp[1]), c.W,
+                                   ERef{et(c.Eh[0]), et(c.Eh[1]), et(c.Eh[2])}, ce, c.fe, eb, 1, et(c.E0[0]), et(c.E0[1]), et(c.E0[2]));
+            else if constexpr (D == 2)
                 hipLaunchKernelGGL((k_emf2<true>), dim3(blocks(eb.count())), dim3(256), 0, s, fs(c.Fp[0]), fs(c.Fp[1]), c.W, none, ce, c.fe, eb, 1,
                                    et(c.E0[0]), et(c.E0[1]), et(c.E0[2]));
             else if constexpr (eref_in_sweep<D, MHD, false, D - 1>())  // the reference EMF from the z sweep
@@ -2013,6 +2020,8 @@ __device__ __forceinline__ SweepArgs batch_sweep_args(const BatchArgs& A, const 
     s.lo[2] = 0; s.hi[2] = 1;
     s.Fo = FIN ? v.F[AX] : v.Fp[AX];
     s.stage = (FIN ? ST_FLUX_X : ST_PRED_X) + AX;
+    if constexpr (eref_in_sweep<2, true, FIN, AX>())
+        for (int q = 0; q < 3; q++) s.eref[q] = v.Eh[q];  // the predictor EMF's reference, before whalf overwrites it
     return s;
 }
 template <bool FIN>
@@ -2045,8 +2054,13 @@ KB __global__ void kb_emf(BatchArgs A, BlockTable bt) {
     const int lo = PRED ? -1 : 0, ex = PRED ? 2 : 1;
     const Box eb = make_box(2, lo, v.ce.n[0] + ex, lo, v.ce.n[1] + ex);
     if (PRED) {
-        const ERef none{nullptr, nullptr, nullptr};
-        emf2_body<true>(v.Fp[0], v.Fp[1], v.W, none, v.ce, v.fe, eb, 1, v.E0[0], v.E0[1], v.E0[2], t);
+        if constexpr (eref_in_sweep<2, true, false, 1>()) {  // the reference EMF from the y sweep
+            const ERef eh{v.Eh[0], v.Eh[1], v.Eh[2]};
+            emf2_body<false>(v.Fp[0], v.Fp[1], v.W, eh, v.ce, v.fe, eb, 1, v.E0[0], v.E0[1], v.E0[2], t);
+        } else {
+            const ERef none{nullptr, nullptr, nullptr};
+            emf2_body<true>(v.Fp[0], v.Fp[1], v.W, none, v.ce, v.fe, eb, 1, v.E0[0], v.E0[1], v.E0[2], t);
+        }
     } else {
         const ERef eh{v.Eh[0], v.Eh[1], v.Eh[2]};
         emf2_body<false>(v.F[0], v.F[1], v.W, eh, v.ce, v.fe, eb, 1, v.E0[0], v.E0[1], v.E0[2], t);
