@@ -988,7 +988,8 @@ struct dg_batch {
     double* dt_last = nullptr;
     long *steps = nullptr, *fails = nullptr;
     int* status = nullptr;
-    unsigned long long *err = nullptr, *vmax = nullptr;
+    unsigned long long *err = nullptr, *vmax = nullptr, *vnext = nullptr;
+    int* cfl_ok = nullptr;
     double* dtp = nullptr;
     int *halv = nullptr, *active = nullptr;
     double* stage = nullptr;                        // upload/download staging, largest patch
@@ -1087,6 +1088,7 @@ static BatchArgs batch_args(const dg_batch* b, int commit, int cls_filter) {
     a.Fp = b->Fp; a.F = b->F; a.Bh = b->Bh; a.E0 = b->E0; a.Eh = b->Eh;
     a.parity = b->parity; a.dt_last = b->dt_last; a.steps = b->steps; a.fails = b->fails; a.status = b->status;
     a.err = b->err; a.vmax = b->vmax; a.dt = b->dtp; a.halv = b->halv; a.active = b->active;
+    a.vnext = b->vnext; a.cfl_ok = b->cfl_ok;
     a.ph = b->ph;
     a.cfl_coeff = b->d.cfl_coeff; a.tol = b->d.timestep_tol > 0 ? b->d.timestep_tol : 1e-14;
     a.cfl_mode = b->d.cfl_mode; a.bc_kind = b->bc_kind;
@@ -1146,7 +1148,8 @@ int dg_batch_create(const dg_desc* physics, int npatch, const int* nx, const int
          alloc((void**)&b->fails, sizeof(long) * n) && alloc((void**)&b->status, sizeof(int) * n) &&
          alloc((void**)&b->err, sizeof(unsigned long long) * n) && alloc((void**)&b->vmax, sizeof(unsigned long long) * n) &&
          alloc((void**)&b->dtp, sizeof(double) * n) && alloc((void**)&b->halv, sizeof(int) * n) &&
-         alloc((void**)&b->active, sizeof(int) * n) &&
+         alloc((void**)&b->active, sizeof(int) * n) && alloc((void**)&b->vnext, sizeof(unsigned long long) * n) &&
+         alloc((void**)&b->cfl_ok, sizeof(int) * n) &&
          alloc((void**)&b->stage, sizeof(double) * std::max(8 * b->max_nc, 3 * b->max_nfc));
     ok = ok && hipMemcpyAsync(b->dP, b->P.data(), sizeof(PatchDesc) * n, hipMemcpyHostToDevice, b->s) == hipSuccess;
     // set-up only (never inside a captured graph)
@@ -1159,6 +1162,8 @@ int dg_batch_create(const dg_desc* physics, int npatch, const int* nx, const int
     ok = ok && hipMemsetAsync(b->status, 0, sizeof(int) * n, b->s) == hipSuccess;
     ok = ok && hipMemsetAsync(b->halv, 0, sizeof(int) * n, b->s) == hipSuccess;
     ok = ok && hipMemsetAsync(b->active, 0, sizeof(int) * n, b->s) == hipSuccess;
+    ok = ok && hipMemsetAsync(b->vnext, 0, sizeof(unsigned long long) * n, b->s) == hipSuccess;
+    ok = ok && hipMemsetAsync(b->cfl_ok, 0, sizeof(int) * n, b->s) == hipSuccess;
     ok = ok && hipEventCreate(&b->e0) == hipSuccess && hipEventCreate(&b->e1) == hipSuccess;
     ok = ok && hipStreamSynchronize(b->s) == hipSuccess;
     for (int k = 0; ok && k < BK_COUNT; k++) ok = batch_table(b, k) == DG_OK;
@@ -1176,7 +1181,7 @@ int dg_batch_destroy(dg_batch* b) {
     if (b->s && hipStreamSynchronize(b->s) != hipSuccess) rc = DG_ERR_HIP;
     batch_drop_graph(b);
     void* bufs[] = {b->W[0], b->W[1], b->B[0], b->B[1], b->Fp, b->F, b->Bh, b->E0, b->Eh, b->dP, b->parity, b->dt_last,
-                    b->steps, b->fails, b->status, b->err, b->vmax, b->dtp, b->halv, b->active, b->stage};
+                    b->steps, b->fails, b->status, b->err, b->vmax, b->dtp, b->halv, b->active, b->stage, b->vnext, b->cfl_ok};
     for (void* p : bufs)
         if (p && hipFree(p) != hipSuccess) rc = DG_ERR_HIP;
     for (int k = 0; k < BK_COUNT; k++)
@@ -1208,6 +1213,7 @@ int dg_batch_upload(dg_batch* b, int patch, const double* w, int stride, const d
     const long lzero = 0;
     BCHK(hipMemcpyAsync(b->status + patch, &zero, sizeof(int), hipMemcpyHostToDevice, b->s));
     BCHK(hipMemcpyAsync(b->halv + patch, &zero, sizeof(int), hipMemcpyHostToDevice, b->s));
+    BCHK(hipMemcpyAsync(b->cfl_ok + patch, &zero, sizeof(int), hipMemcpyHostToDevice, b->s));  // a new state: full CFL next
     BCHK(hipMemcpyAsync(b->steps + patch, &lzero, sizeof(long), hipMemcpyHostToDevice, b->s));
     BCHK(hipMemcpyAsync(b->fails + patch, &lzero, sizeof(long), hipMemcpyHostToDevice, b->s));
     BCHK(hipStreamSynchronize(b->s));
@@ -1233,6 +1239,7 @@ int dg_batch_download(dg_batch* b, int patch, double* w, int stride, double* B) 
 
 int dg_batch_init_B(dg_batch* b) {
     if (!b) return DG_ERR_ARG;
+    BCHK(hipMemsetAsync(b->cfl_ok, 0, sizeof(int) * b->npatch, b->s));  // the cells' B change: full CFL next (not captured)
     batch_launch(b, 1, -1, true);
     BCHK(hipGetLastError());
     BCHK(hipStreamSynchronize(b->s));

@@ -1925,7 +1925,11 @@ KB __global__ void kb_begin(BatchArgs A) {
     const bool on = A.status[p] == 0 && (A.cls_filter < 0 || A.P[p].cls == A.cls_filter);
     A.active[p] = on ? 1 : 0;
     if (on) {
-        A.vmax[p] = 0ull;
+        // the interior's CFL maximum from the apply pass that wrote the
+        // current state, when it did; kb_cfl adds the ghost shell (or, without
+        // it, every cell)
+        A.vmax[p] = A.cfl_ok[p] ? A.vnext[p] : 0ull;
+        A.vnext[p] = 0ull;
         A.err[p] = ~0ull;
     }
 }
@@ -1975,8 +1979,10 @@ KB __global__ void kb_cfl(BatchArgs A, BlockTable bt) {
     const PatchView v = patch_view(A, p);
     const long ny2 = v.ce.n[1] + 2;
     double m = 0;
-    if (t < (long)(v.ce.n[0] + 2) * ny2)
-        m = dmax(m, cfl_speed_at<2, true>(v.W, v.ce, (int)(t / ny2) - 1, (int)(t % ny2) - 1, 0, A.ph, A.cfl_mode, v.dx, v.dy, 1.0));
+    const int i = (int)(t / ny2) - 1, j = (int)(t % ny2) - 1;
+    const bool interior = i >= 0 && i < v.ce.n[0] && j >= 0 && j < v.ce.n[1];
+    if (t < (long)(v.ce.n[0] + 2) * ny2 && !(interior && A.cfl_ok[p]))
+        m = dmax(m, cfl_speed_at<2, true>(v.W, v.ce, i, j, 0, A.ph, A.cfl_mode, v.dx, v.dy, 1.0));
     for (int off = 32; off > 0; off >>= 1) m = dmax(m, __shfl_xor(m, off));
     __shared__ double wmax[BATCH_THREADS / 64];
     if ((threadIdx.x & 63) == 0) wmax[threadIdx.x >> 6] = m;
@@ -2087,12 +2093,26 @@ KB __global__ void kb_apply(BatchArgs A, BlockTable bt) {
     BATCH_ITEM(bt)
     const PatchView v = patch_view(A, p);
     const double dt = A.dt[p];
-    apply_body<2, true>(v.W, v.Wn, v.F[0], v.F[1], v.F[2], v.Bn[0], v.Bn[1], v.Bn[2], v.ce, v.fe, dt / v.dx, dt / v.dy, dt / 1.0, A.ph,
-                        A.err + p, t);
+    double m = apply_body<2, true>(v.W, v.Wn, v.F[0], v.F[1], v.F[2], v.Bn[0], v.Bn[1], v.Bn[2], v.ce, v.fe, dt / v.dx, dt / v.dy,
+                                   dt / 1.0, A.ph, A.err + p, t, true, A.cfl_mode, v.dx, v.dy, 1.0);
+    // the new state's CFL maximum over the patch's interior, for the next
+    // step if this one is committed (one atomic per workgroup)
+    for (int off = 32; off > 0; off >>= 1) m = dmax(m, __shfl_xor(m, off));
+    __shared__ double wmax[BATCH_THREADS / 64];
+    if ((threadIdx.x & 63) == 0) wmax[threadIdx.x >> 6] = m;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double b = wmax[0];
+        for (int q = 1; q < BATCH_THREADS / 64; q++) b = dmax(b, wmax[q]);
+        if (b > 0) atomicMax(A.vnext + p, (unsigned long long)__double_as_longlong(b));
+    }
 }
 KB __global__ void kb_commit(BatchArgs A) {
     const int p = (int)tid();
-    if (p >= A.npatch || !A.active[p] || !A.commit) return;
+    if (p >= A.npatch || !A.active[p]) return;
+    // vnext describes the state only when the attempt was committed
+    A.cfl_ok[p] = A.commit && A.err[p] == ~0ull ? 1 : 0;
+    if (!A.commit) return;
     if (A.err[p] == ~0ull) {
         A.parity[p] = 1 - A.parity[p];
         A.dt_last[p] = A.dt[p];

@@ -106,6 +106,8 @@ struct BatchArgs {
     int* status;               // per patch: 0, or the DG_* code that stopped it
     unsigned long long* err;   // per patch failure key of the current attempt
     unsigned long long* vmax;  // per patch CFL signal speed (FP64 bits)
+    unsigned long long* vnext; // per patch: the apply pass's maximum over the interior of the state it wrote (FP64 bits)
+    int* cfl_ok;               // per patch: vnext describes the current state (its last attempt was committed)
     double* dt;                // per patch dt of the current attempt
     int* halv;                 // per patch halvings since the last accepted step
     int* active;               // per patch: steps in this launch sequence
