@@ -476,21 +476,23 @@ struct SweepArgs {
     ET* eref[3];           // FP32 form, 3D MHD predictor z sweep: the cells' reference EMF cross(B, v) (else unused)
 };
 // The FP32 form's MHD predictor sweep along the contiguous axis also writes
-// each rebuilt cell's reference EMF, cross(B, v) of the state, rounded to
+// each rebuilt cell's reference EMF, cross(B, v) of the state it holds in
 // FP32: its cells cover the predictor EMF's ([-2, n+1] in every dimension),
 // which then reads 12 bytes per cell instead of B and v in FP64 (48 bytes),
 // and in 2D forms each cell's once instead of once per corner.
 template <int D, bool MHD, bool FINAL, int AX> constexpr bool eref_in_sweep() {
     return D >= 2 && AX == D - 1 && MHD && !FINAL && sizeof(FT) == 4;
 }
-// A cell rebuilt by two lanes (segment or wavefront overlaps) is written
-// twice, with the same bits.
-__device__ __forceinline__ void store_eref(const SweepArgs& a, bool live, const int* idx) {
+// From the cell state the sweep already holds in FP32.  A cell rebuilt by
+// two lanes (segment or wavefront overlaps) is written twice, with the same
+// bits.
+template <class R>
+__device__ __forceinline__ void store_eref(const SweepArgs& a, bool live, const int* idx, const P<R>& w) {
     if (!live || !a.eref[0]) return;
-    const long m = a.ce.at(idx[0], idx[1], idx[2]), e = a.fe.at(idx[0], idx[1], idx[2]);
-    double x, y, z;
-    cross(a.W(F_BX, m), a.W(F_BY, m), a.W(F_BZ, m), a.W(F_VX, m), a.W(F_VY, m), a.W(F_VZ, m), x, y, z);
-    a.eref[0][e] = ET(x); a.eref[1][e] = ET(y); a.eref[2][e] = ET(z);
+    const long e = a.fe.at(idx[0], idx[1], idx[2]);
+    a.eref[0][e] = ET(w.By * w.vz - w.Bz * w.vy);  // cross(B, v), Arithmetic.cpp:359-365
+    a.eref[1][e] = ET(w.Bz * w.vx - w.Bx * w.vz);
+    a.eref[2][e] = ET(w.Bx * w.vy - w.By * w.vx);
 }
 
 // Conversions of the CTU-corrected half states.  The FP64 build keeps the
@@ -656,10 +658,11 @@ __device__ __forceinline__ void cell_states_from(const SweepArgs& a, int i, int 
 
 // The same, reading the three cells.
 template <int D, bool MHD, class R, int AX, bool FINAL>
-__device__ __forceinline__ void cell_states(const SweepArgs& a, int i, int j, int k, P<R>& L, P<R>& Rr) {
+__device__ __forceinline__ void cell_states(const SweepArgs& a, int i, int j, int k, P<R>& L, P<R>& Rr, P<R>* wc_out = nullptr) {
     const long m = a.ce.at(i, j, k);
     const int c = AX == 0 ? i : AX == 1 ? j : k;
     const P<R> wc = load_p<R>(a.W, m, MHD);
+    if (wc_out) *wc_out = wc;
     if (c >= -a.ce.g + 1 && c < a.ce.n[AX] + a.ce.g - 1) {
         const long off = a.ce.s[AX];
         const P<R> wl = load_p<R>(a.W, m - off, MHD), wr = load_p<R>(a.W, m + off, MHD);
@@ -855,10 +858,10 @@ __device__ __forceinline__ void sweep_wave_body(const SweepArgs& a, long seg_id,
     idx[T0] = a.lo[T0] + r;
     const int c = a.lo[AX] + seg * (SW - 1) - 1 + ll;
     const bool live = c < a.hi[AX];
-    P<R> L = pzero<R>(), Rr = pzero<R>();
+    P<R> L = pzero<R>(), Rr = pzero<R>(), wc = pzero<R>();
     idx[AX] = c;
-    if (live) cell_states<D, MHD, R, AX, FINAL>(a, idx[0], idx[1], idx[2], L, Rr);
-    if constexpr (eref_in_sweep<D, MHD, FINAL, AX>()) store_eref(a, live, idx);
+    if (live) cell_states<D, MHD, R, AX, FINAL>(a, idx[0], idx[1], idx[2], L, Rr, &wc);
+    if constexpr (eref_in_sweep<D, MHD, FINAL, AX>()) store_eref(a, live, idx, wc);
     const P<R> Rl = shfl_up1(Rr);  // lane 0 of a segment takes the lane below's value and leaves it unused
     if (ll >= 1 && live) face_flux<MHD, R, AX>(a, Rl, L, a.ce.at(idx[0], idx[1], idx[2]));
 }
@@ -885,9 +888,9 @@ __device__ __forceinline__ void sweep_stream_body(const SweepArgs& a, int w, int
     if (D == 3) { idx[0] = a.lo[0] + (int)(line / (unsigned)n1); idx[1] = a.lo[1] + (int)(line % (unsigned)n1); }
     if (D == 2) idx[0] = a.lo[0] + (int)line;
     idx[AX] = a.lo[AX] - 1 + (int)c;
-    P<R> L = pzero<R>(), Rr = pzero<R>();
-    if (live) cell_states<D, MHD, R, AX, FINAL>(a, idx[0], idx[1], idx[2], L, Rr);
-    if constexpr (eref_in_sweep<D, MHD, FINAL, AX>()) store_eref(a, live, idx);
+    P<R> L = pzero<R>(), Rr = pzero<R>(), wc = pzero<R>();
+    if (live) cell_states<D, MHD, R, AX, FINAL>(a, idx[0], idx[1], idx[2], L, Rr, &wc);
+    if constexpr (eref_in_sweep<D, MHD, FINAL, AX>()) store_eref(a, live, idx, wc);
     const P<R> Rl = shfl_up1(Rr);  // lane 0 takes the lane below's value and leaves it unused
     if (lane >= 1 && live && c >= 1) face_flux<MHD, R, AX>(a, Rl, L, a.ce.at(idx[0], idx[1], idx[2]));
 }
