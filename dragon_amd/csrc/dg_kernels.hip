@@ -473,15 +473,17 @@ struct SweepArgs {
     Phys ph;
     unsigned long long* err;
     int stage, chunk;
-    ET* eref[3];           // FP32 form, 3D MHD predictor z sweep: the cells' reference EMF cross(B, v) (else unused)
+    ET* eref[3];           // MHD predictor sweep along the contiguous axis: the cells' reference EMF cross(B, v) (else unused)
 };
-// The FP32 form's MHD predictor sweep along the contiguous axis also writes
-// each rebuilt cell's reference EMF, cross(B, v) of the state it holds in
-// FP32: its cells cover the predictor EMF's ([-2, n+1] in every dimension),
-// which then reads 12 bytes per cell instead of B and v in FP64 (48 bytes),
-// and in 2D forms each cell's once instead of once per corner.
+// The MHD predictor sweep along the contiguous axis also writes each rebuilt
+// cell's reference EMF, cross(B, v) of the state it holds: its cells cover
+// the predictor EMF's ([-2, n+1] in every dimension). In the FP32 form the
+// EMF then reads 12 bytes per cell instead of B and v in FP64 (48 bytes), and
+// in 2D forms each cell's once instead of once per corner. The FP64 form does
+// this in 3D only (the EMF reads 24 bytes instead of 48); its cross product is
+// the one the EMF would form from the same W, so results are unchanged.
 template <int D, bool MHD, bool FINAL, int AX> constexpr bool eref_in_sweep() {
-    return D >= 2 && AX == D - 1 && MHD && !FINAL && sizeof(FT) == 4;
+    return D >= 2 && AX == D - 1 && MHD && !FINAL && (sizeof(FT) == 4 || D == 3);
 }
 // From the cell state the sweep already holds in FP32.  A cell rebuilt by
 // two lanes (segment or wavefront overlaps) is written twice, with the same
