@@ -1065,24 +1065,19 @@ KB __global__ void k_emf3(FSoa FX, FSoa FY, FSoa FZ, Soa W, ERef ref, Ext ce, Ex
 
 // The same edges blocked for reuse: a workgroup owns a (j,k) tile of
 // TJ x TK edges and walks a run of i-planes.  Each cell's operands (9 flux
-// components and its reference EMF, computed once) are read from memory once
-// per plane into LDS, with a one-cell halo on the low j and k sides; the
-// previous plane stays in the other LDS buffer for the i-1 neighbours.  Every
-// edge is evaluated by emf3_eval, as in k_emf3.
+// components and its stored reference EMF) are read from memory once per
+// plane into LDS, with a one-cell halo on the low j and k sides; the previous
+// plane stays in the other LDS buffer for the i-1 neighbours.  Every edge is
+// evaluated by emf3_eval, as in k_emf3.
 //
-// Two tile shapes.  With PF the tile and its halo are one cell per thread
-// (3 x 63 edges, 4 x 64 cells), and each thread reads the next plane's cell
-// into registers before the current plane's edges are evaluated, so the
-// reads' latency overlaps that work.  Without it the tile is 4 x 64 edges
-// (5 x 65 cells, fewer halo reads per edge) and each plane is read, then
-// evaluated: the predictor EMF of the FP32 form, which reads the cells' B and
-// v in FP64 to form the reference EMF, is bound by those bytes and runs 3%
-// faster so; the other three instances run 4-12% faster with PF.
-template <bool PF> struct EmfTile {
-    static constexpr int TK = PF ? 63 : 64, TJ = PF ? 3 : 4, CELLS = (TJ + 1) * (TK + 1);
+// The tile and its halo are one cell per thread (3 x 63 edges, 4 x 64
+// cells), and each thread reads the next plane's cell into registers before
+// the current plane's edges are evaluated, so the reads' latency overlaps
+// that work.
+struct EmfTile {
+    static constexpr int TK = 63, TJ = 3, CELLS = (TJ + 1) * (TK + 1);
 };
-static_assert(EmfTile<true>::CELLS == 256, "one halo-extended tile cell per thread");
-template <bool FROM_W> constexpr bool emf_prefetch() { return !(FROM_W && sizeof(FT) == 4); }
+static_assert(EmfTile::CELLS == 256, "one halo-extended tile cell per thread");
 // emf3_eval reads the plane below the edge (offset a = -1) only through the
 // y and z fluxes' mass and first B slots and the reference EMF's y and z
 // components; those are kept for two planes (ring slot = plane & 1), the
@@ -1098,11 +1093,9 @@ template <int CELLS> struct EmfPlanes {
 // FP32 operands: five workgroups per CU fit in LDS (23-30 KB each), and at
 // most 96 registers keep five waves per SIMD for them.
 template <class T> constexpr int emf_tiled_waves() { return sizeof(T) == 4 ? 5 : 1; }
-template <bool FROM_W>
 KB __attribute__((amdgpu_waves_per_eu(emf_tiled_waves<FT>()))) __global__ void k_emf3_tiled(
-    FSoa FX, FSoa FY, FSoa FZ, Soa W, ERef ref, Ext ce, Ext fe, Box box, int upwind, int iplanes, ET* Ex, ET* Ey, ET* Ez) {
-    constexpr bool PF = emf_prefetch<FROM_W>();
-    constexpr int TK = EmfTile<PF>::TK, TJ = EmfTile<PF>::TJ, CELLS = EmfTile<PF>::CELLS;
+    FSoa FX, FSoa FY, FSoa FZ, ERef ref, Ext ce, Ext fe, Box box, int upwind, int iplanes, ET* Ex, ET* Ey, ET* Ez) {
+    constexpr int TK = EmfTile::TK, TJ = EmfTile::TJ, CELLS = EmfTile::CELLS;
     __shared__ EmfPlanes<CELLS> S;
     const int nkt = (box.hi[2] - box.lo[2] + TK - 1) / TK;
     const int njt = (box.hi[1] - box.lo[1] + TJ - 1) / TJ;
@@ -1119,7 +1112,7 @@ KB __attribute__((amdgpu_waves_per_eu(emf_tiled_waves<FT>()))) __global__ void k
     // edge reads them.
     struct CellVals {
         FT f[9];      // FY slots 0, 1, FZ slots 0, 1, FX slots 0, 1, 2, FY slot 2, FZ slot 2
-        double w[6];  // B and v of the cell (FROM_W), or the reference EMF's x, y, z
+        double w[3];  // the reference EMF's x, y, z
     };
     auto in_box = [&](int c) { return j0 + c / (TK + 1) - 1 < box.hi[1] && k0 + c % (TK + 1) - 1 < box.hi[2]; };
     auto fetch = [&](int c, int i, bool own, CellVals& v) {
@@ -1135,13 +1128,8 @@ KB __attribute__((amdgpu_waves_per_eu(emf_tiled_waves<FT>()))) __global__ void k
             v.f[8] = FZ(emf_fld(2, 2), q);
         }
         if (upwind) {
-            if (FROM_W) {
-                v.w[0] = W(F_BX, q); v.w[1] = W(F_BY, q); v.w[2] = W(F_BZ, q);
-                v.w[3] = W(F_VX, q); v.w[4] = W(F_VY, q); v.w[5] = W(F_VZ, q);
-            } else {
-                const long e = fe.at(i, cj, ck);
-                v.w[0] = ed(ref.x, e); v.w[1] = ed(ref.y, e); v.w[2] = ed(ref.z, e);
-            }
+            const long e = fe.at(i, cj, ck);
+            v.w[0] = ed(ref.x, e); v.w[1] = ed(ref.y, e); v.w[2] = ed(ref.z, e);
         }
     };
     auto stash = [&](int c, int i, bool own, const CellVals& v) {
@@ -1150,44 +1138,24 @@ KB __attribute__((amdgpu_waves_per_eu(emf_tiled_waves<FT>()))) __global__ void k
         if (own)
             for (int f = 0; f < 5; f++) S.fo[f][c] = v.f[4 + f];
         if (upwind) {
-            double x = v.w[0], y = v.w[1], z = v.w[2];
-            if (FROM_W) cross(v.w[0], v.w[1], v.w[2], v.w[3], v.w[4], v.w[5], x, y, z);
-            S.rk[rs][0][c] = y;
-            S.rk[rs][1][c] = z;
-            if (own) S.rx[c] = x;
-        }
-    };
-    auto load = [&](int i, bool own) {  // the whole plane at once (no prefetch)
-        for (int c = threadIdx.x; c < CELLS; c += blockDim.x) {
-            if (!in_box(c)) continue;
-            CellVals v;
-            fetch(c, i, own, v);
-            stash(c, i, own, v);
+            S.rk[rs][0][c] = v.w[1];
+            S.rk[rs][1][c] = v.w[2];
+            if (own) S.rx[c] = v.w[0];
         }
     };
     const int tk = threadIdx.x % TK, tj = threadIdx.x / TK;
     const int j = j0 + tj, k = k0 + tk;
     const bool live = tj < TJ && j < box.hi[1] && k < box.hi[2];
     auto cell = [&](int dj, int dk) { return (tj + dj + 1) * (TK + 1) + (tk + dk + 1); };
-    const int mc = threadIdx.x;  // PF: this thread's cell
-    const bool mine = PF && in_box(mc);
+    const int mc = threadIdx.x;  // this thread's cell
+    const bool mine = in_box(mc);
     CellVals v;
-    if constexpr (PF) {
-        if (mine) { fetch(mc, ib - 1, false, v); stash(mc, ib - 1, false, v); }
-        if (mine) fetch(mc, ib, true, v);
-    } else {
-        load(ib - 1, false);
-    }
+    if (mine) { fetch(mc, ib - 1, false, v); stash(mc, ib - 1, false, v); }
+    if (mine) fetch(mc, ib, true, v);
     for (int i = ib; i < ie; i++) {
-        if constexpr (PF) {
-            if (mine) stash(mc, i, true, v);
-        } else {
-            load(i, true);
-        }
+        if (mine) stash(mc, i, true, v);
         __syncthreads();
-        if constexpr (PF) {
-            if (mine && i + 1 < ie) fetch(mc, i + 1, true, v);  // waited for at the next stash, after this plane's edges
-        }
+        if (mine && i + 1 < ie) fetch(mc, i + 1, true, v);  // waited for at the next stash, after this plane's edges
         auto ring = [&](int a) { return (i + a) & 1; };  // a = 0 or -1
         // the x fluxes and the reference EMF's x component are read at a = 0 only (emf3_eval)
         auto fx = [&](int f, int, int b2, int c2) { return double(S.fo[f][cell(b2, c2)]); };
@@ -1206,15 +1174,14 @@ KB __attribute__((amdgpu_waves_per_eu(emf_tiled_waves<FT>()))) __global__ void k
 
 // i-planes per workgroup of the tiled EMF (one extra plane is read per run)
 constexpr int EMF_IPLANES = 32;
-template <bool FROM_W>
-static void emf3_tiled(const FSoa& FX, const FSoa& FY, const FSoa& FZ, const Soa& W, const ERef& ref, const Ext& ce, const Ext& fe,
-                       const Box& eb, int upwind, ET* Ex, ET* Ey, ET* Ez, hipStream_t s) {
-    using T = EmfTile<emf_prefetch<FROM_W>()>;
+static void emf3_tiled(const FSoa& FX, const FSoa& FY, const FSoa& FZ, const ERef& ref, const Ext& ce, const Ext& fe, const Box& eb,
+                       int upwind, ET* Ex, ET* Ey, ET* Ez, hipStream_t s) {
+    using T = EmfTile;
     const long nkt = (eb.hi[2] - eb.lo[2] + T::TK - 1) / T::TK, njt = (eb.hi[1] - eb.lo[1] + T::TJ - 1) / T::TJ;
     const long nic = (eb.hi[0] - eb.lo[0] + EMF_IPLANES - 1) / EMF_IPLANES;
     if (nkt <= 0 || njt <= 0 || nic <= 0) return;
-    hipLaunchKernelGGL((k_emf3_tiled<FROM_W>), dim3((unsigned)(nkt * njt * nic)), dim3(256), 0, s, FX, FY, FZ, W, ref, ce, fe, eb, upwind,
-                       EMF_IPLANES, Ex, Ey, Ez);
+    hipLaunchKernelGGL(k_emf3_tiled, dim3((unsigned)(nkt * njt * nic)), dim3(256), 0, s, FX, FY, FZ, ref, ce, fe, eb, upwind, EMF_IPLANES,
+                       Ex, Ey, Ez);
 }
 
 // Faraday (B_fields.cpp:21-66) fused with the clone: Bout = Bin everywhere,
@@ -1676,11 +1643,11 @@ static void step_fused(const StepCtx& c, hipStream_t s) {
             else if constexpr (D == 2)
                 hipLaunchKernelGGL((k_emf2<true>), dim3(blocks(eb.count())), dim3(256), 0, s, fs(c.Fp[0]), fs(c.Fp[1]), c.W, none, ce, c.fe, eb, 1,
                                    et(c.E0[0]), et(c.E0[1]), et(c.E0[2]));
-            else if constexpr (eref_in_sweep<D, MHD, false, D - 1>())  // the reference EMF from the z sweep
-                emf3_tiled<false>(fs(c.Fp[0]), fs(c.Fp[1]), fs(c.Fp[2]), c.W, ERef{et(c.Eh[0]), et(c.Eh[1]), et(c.Eh[2])}, ce, c.fe, eb, 1,
-                                  et(c.E0[0]), et(c.E0[1]), et(c.E0[2]), s);
-            else
-                emf3_tiled<true>(fs(c.Fp[0]), fs(c.Fp[1]), fs(c.Fp[2]), c.W, none, ce, c.fe, eb, 1, et(c.E0[0]), et(c.E0[1]), et(c.E0[2]), s);
+            else {  // the reference EMF from the z sweep
+                static_assert(eref_in_sweep<D, MHD, false, D - 1>(), "the 3D predictor sweep stores the reference EMF");
+                emf3_tiled(fs(c.Fp[0]), fs(c.Fp[1]), fs(c.Fp[2]), ERef{et(c.Eh[0]), et(c.Eh[1]), et(c.Eh[2])}, ce, c.fe, eb, 1, et(c.E0[0]),
+                           et(c.E0[1]), et(c.E0[2]), s);
+            }
             mark(c, K_EMF_PRED);
             hipLaunchKernelGGL((k_faraday<D, ET>), dim3(blocks(c.fe.size)), dim3(256), 0, s, et(c.E0[0]), et(c.E0[1]), et(c.E0[2]), c.B[0], c.B[1],
                                c.B[2], et(c.Bh[0]), et(c.Bh[1]), et(c.Bh[2]), c.fe, 1, 0.5 * dt_d[0], 0.5 * dt_d[1], 0.5 * dt_d[2]);
@@ -1705,7 +1672,7 @@ static void step_fused(const StepCtx& c, hipStream_t s) {
             hipLaunchKernelGGL((k_emf2<false>), dim3(blocks(eb.count())), dim3(256), 0, s, fs(c.F[0]), fs(c.F[1]), c.W, eh, ce, c.fe, eb, c.ctu,
                                et(c.E0[0]), et(c.E0[1]), et(c.E0[2]));
         else
-            emf3_tiled<false>(fs(c.F[0]), fs(c.F[1]), fs(c.F[2]), c.W, eh, ce, c.fe, eb, c.ctu, et(c.E0[0]), et(c.E0[1]), et(c.E0[2]), s);
+            emf3_tiled(fs(c.F[0]), fs(c.F[1]), fs(c.F[2]), eh, ce, c.fe, eb, c.ctu, et(c.E0[0]), et(c.E0[1]), et(c.E0[2]), s);
         mark(c, K_EMF);
         hipLaunchKernelGGL((k_faraday<D, double>), dim3(blocks(c.fe.size)), dim3(256), 0, s, et(c.E0[0]), et(c.E0[1]), et(c.E0[2]), c.B[0], c.B[1], c.B[2],
                            c.Bn[0], c.Bn[1], c.Bn[2], c.fe, 0, dt_d[0], dt_d[1], dt_d[2]);

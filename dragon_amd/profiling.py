@@ -37,8 +37,9 @@ import re
 _CLASSES = [
     (r"k_sweep_(march|wave)<\d, (true|false), (double|float), (\d), false>", lambda m: "pred_sweep_" + "xyz"[int(m.group(4))]),
     (r"k_sweep_(march|wave)<\d, (true|false), (double|float), (\d), true>", lambda m: "flux_sweep_" + "xyz"[int(m.group(4))]),
-    (r"k_emf[23](_tiled)?<true>", "emf_pred"),
-    (r"k_emf[23](_tiled)?<false>", "emf"),
+    (r"k_emf[23]<true>", "emf_pred"),
+    (r"k_emf[23]<false>", "emf"),
+    (r"k_emf3_tiled\(", "emf (predictor + final)"),  # one kernel for both EMFs of the 3D step
     (r"k_faraday<", "faraday (half + final)"),
     (r"k_whalf<", "whalf"),
     (r"k_apply1?<", "apply"),

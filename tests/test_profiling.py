@@ -18,6 +18,7 @@ def test_kernel_class_names():
     assert P.kernel_class("void dg::k_sweep_march<3, true, double, 1, false>(dg::SweepArgs)") == "pred_sweep_y"
     assert P.kernel_class("void dg::k_sweep_wave<3, true, float, 2, true>(dg::SweepArgs)") == "flux_sweep_z"
     assert P.kernel_class("void dg::k_emf3<true>(dg::Soa, ...)") == "emf_pred"
+    assert P.kernel_class("dg::p64::k_emf3_tiled(dg::SoaT<double>, ...)") == "emf (predictor + final)"
     assert P.kernel_class("void dg::k_apply<3, true>(...)") == "apply"
     assert P.kernel_class("dg::k_bc_cells(dg::BCArgs)") == "boundary"
 
