@@ -57,6 +57,18 @@ void ctu_hydro3(const StepCtx& c, hipStream_t s);
 #endif
 
 __device__ __forceinline__ long tid() { return (long)blockIdx.x * blockDim.x + threadIdx.x; }
+// Workgroups are dealt to the 8 XCDs in turn.  For passes over consecutive
+// cells whose cells read the next row's fluxes, tid_xcd gives the workgroups
+// of each group of 32 (in launch order) to XCDs so that one XCD holds 4
+// consecutive blocks of cells (about two rows of a 512-cell line): the
+// neighbour rows a block reads are then mostly in its own XCD's L2.  A
+// permutation of the block indices within whole groups of 32; the last,
+// partial group keeps its order.
+__device__ __forceinline__ long tid_xcd() {
+    unsigned b = blockIdx.x;
+    if (b < (gridDim.x & ~31u)) b = (b & ~31u) | ((b & 7u) << 2) | ((b >> 3) & 3u);
+    return (long)b * blockDim.x + threadIdx.x;
+}
 
 template <class R> __device__ __forceinline__ P<R> load_p(const Soa& a, long m, bool mhd) {
     P<R> w;
@@ -1279,7 +1291,7 @@ template <int D>
 KB __global__ void k_whalf(Soa W, FSoa FX, FSoa FY, FSoa FZ, const ET* Bhx, const ET* Bhy, const ET* Bhz, Ext ce,
                            Ext fe, Box box, double hx, double hy, double hz, Phys ph, ET* Ehx, ET* Ehy, ET* Ehz,
                            unsigned long long* err) {
-    whalf_body<D>(W, FX, FY, FZ, Bhx, Bhy, Bhz, ce, fe, box, hx, hy, hz, ph, Ehx, Ehy, Ehz, err, tid());
+    whalf_body<D>(W, FX, FY, FZ, Bhx, Bhy, Bhz, ce, fe, box, hx, hy, hz, ph, Ehx, Ehy, Ehz, err, tid_xcd());
 }
 
 // ctu_sweep_MHD corrections (CTU.cpp:65-129 3D, 162-202 2D), in place.
@@ -1401,7 +1413,7 @@ template <int D, bool MHD>
 KB __global__ void k_apply(Soa W, Soa Wn, FSoa FX, FSoa FY, FSoa FZ, const double* Bnx, const double* Bny, const double* Bnz, Ext ce,
                            Ext fe, double dt_dx, double dt_dy, double dt_dz, Phys ph, unsigned long long* err, double* cflpart,
                            int cfl_mode, double cdx, double cdy, double cdz) {
-    const double v = apply_body<D, MHD>(W, Wn, FX, FY, FZ, Bnx, Bny, Bnz, ce, fe, dt_dx, dt_dy, dt_dz, ph, err, tid(), cflpart != nullptr,
+    const double v = apply_body<D, MHD>(W, Wn, FX, FY, FZ, Bnx, Bny, Bnz, ce, fe, dt_dx, dt_dy, dt_dz, ph, err, tid_xcd(), cflpart != nullptr,
                                         cfl_mode, cdx, cdy, cdz);
     if (cflpart) block_max_store(v, cflpart);  // uniform branch
 }
