@@ -58,15 +58,16 @@ void ctu_hydro3(const StepCtx& c, hipStream_t s);
 
 __device__ __forceinline__ long tid() { return (long)blockIdx.x * blockDim.x + threadIdx.x; }
 // Workgroups are dealt to the 8 XCDs in turn.  For passes over consecutive
-// cells whose cells read the next row's fluxes, tid_xcd gives the workgroups
+// points that read the next row's fluxes or edges, tid_xcd gives the workgroups
 // of each group of 32 (in launch order) to XCDs so that one XCD holds 4
 // consecutive blocks of cells (about two rows of a 512-cell line): the
 // neighbour rows a block reads are then mostly in its own XCD's L2.  A
 // permutation of the block indices within whole groups of 32; the last,
 // partial group keeps its order.
 __device__ __forceinline__ long tid_xcd() {
+    constexpr unsigned R = 4, G = 8 * R;
     unsigned b = blockIdx.x;
-    if (b < (gridDim.x & ~31u)) b = (b & ~31u) | ((b & 7u) << 2) | ((b >> 3) & 3u);
+    if (b < (gridDim.x & ~(G - 1))) b = (b & ~(G - 1)) | ((b & 7u) * R) | ((b >> 3) & (R - 1));
     return (long)b * blockDim.x + threadIdx.x;
 }
 
@@ -1232,7 +1233,7 @@ template <int D, class BO>
 KB __global__ void k_faraday(const ET* Ex, const ET* Ey, const ET* Ez, const double* Bix, const double* Biy,
                              const double* Biz, BO* Box_, BO* Boy, BO* Boz, Ext fe, int gg, double dt_dx, double dt_dy,
                              double dt_dz) {
-    faraday_body<D, BO>(Ex, Ey, Ez, Bix, Biy, Biz, Box_, Boy, Boz, fe, gg, dt_dx, dt_dy, dt_dz, tid());
+    faraday_body<D, BO>(Ex, Ey, Ez, Bix, Biy, Biz, Box_, Boy, Boz, fe, gg, dt_dx, dt_dy, dt_dz, tid_xcd());
 }
 
 // computeBodyField + shouldProtectThermal (B_fields.cpp:70-114, production
