@@ -1111,12 +1111,18 @@ KB __attribute__((amdgpu_waves_per_eu(emf_tiled_waves<FT>()))) __global__ void k
     constexpr int TK = EmfTile::TK, TJ = EmfTile::TJ, CELLS = EmfTile::CELLS;
     __shared__ EmfPlanes<CELLS> S;
     const int nkt = (box.hi[2] - box.lo[2] + TK - 1) / TK;
-    const int njt = (box.hi[1] - box.lo[1] + TJ - 1) / TJ;
-    int b = blockIdx.x;
+    const int njt = (box.hi[1] - box.lo[1] + TJ - 1) / TJ, njq = (njt + 3) / 4;
+    // runs of 4 j-neighbour tiles at one k, each run on one XCD (tid_xcd's
+    // order), so that the low-j halo row a tile reads is mostly its
+    // neighbour's own row in the same L2
+    int b = (int)(tid_xcd() / blockDim.x);
+    const int jq = b % 4;
+    b /= 4;
     const int kt = b % nkt;
     b /= nkt;
-    const int jt = b % njt;
-    const int ic = b / njt;
+    const int jt = (b % njq) * 4 + jq;
+    const int ic = b / njq;
+    if (jt >= njt) return;  // the whole workgroup
     const int k0 = box.lo[2] + kt * TK, j0 = box.lo[1] + jt * TJ;
     const int ib = box.lo[0] + ic * iplanes, ie = min(box.hi[0], ib + iplanes);
     // a cell c of a plane: read into registers (fetch), then into LDS
@@ -1190,7 +1196,7 @@ constexpr int EMF_IPLANES = 32;
 static void emf3_tiled(const FSoa& FX, const FSoa& FY, const FSoa& FZ, const ERef& ref, const Ext& ce, const Ext& fe, const Box& eb,
                        int upwind, ET* Ex, ET* Ey, ET* Ez, hipStream_t s) {
     using T = EmfTile;
-    const long nkt = (eb.hi[2] - eb.lo[2] + T::TK - 1) / T::TK, njt = (eb.hi[1] - eb.lo[1] + T::TJ - 1) / T::TJ;
+    const long nkt = (eb.hi[2] - eb.lo[2] + T::TK - 1) / T::TK, njt = ((eb.hi[1] - eb.lo[1] + T::TJ - 1) / T::TJ + 3) / 4 * 4;  // runs of 4
     const long nic = (eb.hi[0] - eb.lo[0] + EMF_IPLANES - 1) / EMF_IPLANES;
     if (nkt <= 0 || njt <= 0 || nic <= 0) return;
     hipLaunchKernelGGL(k_emf3_tiled, dim3((unsigned)(nkt * njt * nic)), dim3(256), 0, s, FX, FY, FZ, ref, ce, fe, eb, upwind, EMF_IPLANES,
