@@ -64,12 +64,13 @@ __device__ __forceinline__ long tid() { return (long)blockIdx.x * blockDim.x + t
 // neighbour rows a block reads are then mostly in its own XCD's L2.  A
 // permutation of the block indices within whole groups of 32; the last,
 // partial group keeps its order.
-__device__ __forceinline__ long tid_xcd() {
+__device__ __forceinline__ unsigned block_xcd() {
     constexpr unsigned R = 4, G = 8 * R;
     unsigned b = blockIdx.x;
     if (b < (gridDim.x & ~(G - 1))) b = (b & ~(G - 1)) | ((b & 7u) * R) | ((b >> 3) & (R - 1));
-    return (long)b * blockDim.x + threadIdx.x;
+    return b;
 }
+__device__ __forceinline__ long tid_xcd() { return (long)block_xcd() * blockDim.x + threadIdx.x; }
 
 template <class R> __device__ __forceinline__ P<R> load_p(const Soa& a, long m, bool mhd) {
     P<R> w;
@@ -840,7 +841,7 @@ template <int D, int AX, class R, bool FINAL> constexpr int march_min_waves() {
 }
 template <int D, bool MHD, class R, int AX, bool FINAL>
 KS __attribute__((amdgpu_waves_per_eu(march_min_waves<D, AX, R, FINAL>()))) __global__ void k_sweep_march(SweepArgs a) {
-    sweep_march_body<D, MHD, R, AX, FINAL>(a, tid());
+    sweep_march_body<D, MHD, R, AX, FINAL>(a, tid_xcd());
 }
 template <int D, bool MHD, class R, int AX, bool FINAL>
 KS __global__ void k_sweep_march_window(SweepArgs a) { sweep_march_body<D, MHD, R, AX, FINAL, true>(a, tid()); }
@@ -917,7 +918,7 @@ template <int D, bool MHD, class R, int AX, bool FINAL>
 KS __attribute__((amdgpu_waves_per_eu(wave_min_waves<R>()))) __global__ void k_sweep_wave(SweepArgs a) {
     // the wavefront index is uniform: keep it in a scalar register
     const int w = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
-    sweep_stream_body<D, MHD, R, AX, FINAL>(a, (int)blockIdx.x * (int)(blockDim.x >> 6) + w, threadIdx.x & 63);
+    sweep_stream_body<D, MHD, R, AX, FINAL>(a, (int)block_xcd() * (int)(blockDim.x >> 6) + w, threadIdx.x & 63);
 }
 
 // ===================================================== CTU hydro (2D/3D)
@@ -1115,7 +1116,7 @@ KB __attribute__((amdgpu_waves_per_eu(emf_tiled_waves<FT>()))) __global__ void k
     // runs of 4 j-neighbour tiles at one k, each run on one XCD (tid_xcd's
     // order), so that the low-j halo row a tile reads is mostly its
     // neighbour's own row in the same L2
-    int b = (int)(tid_xcd() / blockDim.x);
+    int b = (int)block_xcd();
     const int jq = b % 4;
     b /= 4;
     const int kt = b % nkt;
