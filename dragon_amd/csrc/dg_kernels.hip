@@ -1905,11 +1905,13 @@ __device__ __forceinline__ PatchView patch_view(const BatchArgs& A, int p) {
     v.cls = P.cls;
     return v;
 }
-// workgroup -> (patch, work item); inactive patches' workgroups leave at once
+// workgroup -> (patch, work item), in the XCD order of block_xcd; inactive
+// patches' workgroups leave at once
 #define BATCH_ITEM(bt)                          \
-    const int p = bt.patch[blockIdx.x];         \
+    const unsigned bx = block_xcd();            \
+    const int p = bt.patch[bx];                 \
     if (!A.active[p]) return;                   \
-    const long t = (long)bt.first[blockIdx.x] + threadIdx.x;
+    const long t = (long)bt.first[bx] + threadIdx.x;
 
 KB __global__ void kb_begin(BatchArgs A) {
     const int p = (int)tid();
@@ -2036,7 +2038,7 @@ KS __attribute__((amdgpu_waves_per_eu(wave_min_waves<RT>()))) __global__ void kb
     BATCH_ITEM(bt)  // first is a multiple of the workgroup size: first / 64 + the wavefront in the workgroup
     const PatchView v = patch_view(A, p);
     SweepArgs s = batch_sweep_args<1, FIN>(A, v, p);
-    const long w = (long)bt.first[blockIdx.x] / 64 + __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+    const long w = (long)bt.first[bx] / 64 + __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
     const int lane = threadIdx.x & 63;
     switch (A.ysw[v.cls]) {  // segment width of the patch's size class (uniform per workgroup)
         case 0: s.chunk = 63; sweep_stream_body<2, true, RT, 1, FIN>(s, (int)w, lane); break;  // the patch's columns streamed
